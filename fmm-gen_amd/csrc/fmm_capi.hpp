@@ -1,0 +1,238 @@
+// fmm_capi.hpp -- the extern "C" entry points of include/fmm_amd.h.
+//
+// Included by exactly one translation unit per shared library: the runtime
+// (fmm_runtime.hip) or a generated per-job source.
+#pragma once
+#include <cstdlib>
+
+#include "fmm_host.hpp"
+#include "../../include/fmm_amd.h"
+
+#ifndef FMM_CH
+#define FMM_CH 4
+#endif
+
+struct fmm_plan_s {
+  fmm::Plan p;
+};
+
+namespace fmm {
+
+inline bool bad_dims(int m, int n, int k, int lda, int ldb, int ldc) {
+  if (m < 0 || n < 0 || k < 0) return true;
+  if (lda < std::max(1, m) || ldb < std::max(1, k) || ldc < std::max(1, m)) return true;
+  return false;
+}
+
+inline int run_abc(Plan* P, int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C,
+                   int ldc, void* ws, size_t ws_bytes, hipStream_t s) {
+  if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
+  if (m == 0 || n == 0 || k == 0) return kOK;
+  Prod* dp;
+  Term* dt;
+  int rc = device_tables(P, &dp, &dt);
+  if (rc != kOK) return rc;
+  Prod* ip;
+  Term* it;
+  rc = device_tables(identity_plan(), &ip, &it);
+  if (rc != kOK) return rc;
+  JobList JL;
+  add_fmm_jobs(JL, *P, dp, dt, ip, it, m, n, k, A, lda, B, ldb, C, ldc);
+  const size_t need = ws_ints(JL) * sizeof(int);
+  int* w = static_cast<int*>(ws);
+  bool own = false;
+  if (need > 0 && !w) {
+    if (hipMallocAsync(reinterpret_cast<void**>(&w), need, s) != hipSuccess) return kENoMem;
+    own = true;
+  } else if (need > ws_bytes && w) {
+    return kEInval;
+  }
+  rc = launch_jobs<FMM_CH>(JL, w, s);
+  if (own) (void)hipFreeAsync(w, s);
+  return rc;
+}
+
+inline int run_gemm(int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
+                    hipStream_t s) {
+  if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
+  if (m == 0 || n == 0 || k == 0) return kOK;
+  Prod* ip;
+  Term* it;
+  int rc = device_tables(identity_plan(), &ip, &it);
+  if (rc != kOK) return rc;
+  JobList JL;
+  add_gemm_job(JL, ip, it, m, n, k, A, lda, B, ldb, C, ldc);
+  return launch_jobs<FMM_CH>(JL, nullptr, s);
+}
+
+// Staging buffers of the host-pointer drop-ins (grow-only, one set per
+// process, serialised by a mutex like the reference's global packing pool).
+struct HostStage {
+  std::mutex mu;
+  double* d[3] = {nullptr, nullptr, nullptr};
+  size_t cap[3] = {0, 0, 0};
+  int dev = -1;
+  void release() {
+    for (int i = 0; i < 3; ++i) {
+      if (d[i]) (void)hipFree(d[i]);
+      d[i] = nullptr;
+      cap[i] = 0;
+    }
+  }
+};
+inline HostStage& host_stage() {
+  static HostStage h;
+  return h;
+}
+
+inline int stage_buf(HostStage& h, int i, size_t elems) {
+  if (h.cap[i] >= elems && h.d[i]) return kOK;
+  if (h.d[i]) (void)hipFree(h.d[i]);
+  h.d[i] = nullptr;
+  h.cap[i] = 0;
+  if (hipMalloc(&h.d[i], std::max<size_t>(elems, 1) * sizeof(double)) != hipSuccess) return kENoMem;
+  h.cap[i] = elems;
+  return kOK;
+}
+
+// Copy in, run `body(dA, dB, dC)`, copy C back.
+template <class F>
+inline int host_call(int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
+                     F body) {
+  if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
+  if (m == 0 || n == 0) return kOK;
+  HostStage& h = host_stage();
+  std::lock_guard<std::mutex> g(h.mu);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return kEHip;
+  if (h.dev != dev) {
+    h.release();
+    h.dev = dev;
+  }
+  int rc;
+  if ((rc = stage_buf(h, 0, (size_t)m * std::max(k, 1))) != kOK) return rc;
+  if ((rc = stage_buf(h, 1, (size_t)std::max(k, 1) * n)) != kOK) return rc;
+  if ((rc = stage_buf(h, 2, (size_t)m * n)) != kOK) return rc;
+  const size_t es = sizeof(double);
+  if (k > 0) {
+    if (hipMemcpy2D(h.d[0], m * es, A, (size_t)lda * es, m * es, k, hipMemcpyHostToDevice) != hipSuccess) return kEHip;
+    if (hipMemcpy2D(h.d[1], k * es, B, (size_t)ldb * es, k * es, n, hipMemcpyHostToDevice) != hipSuccess) return kEHip;
+  }
+  if (hipMemcpy2D(h.d[2], m * es, C, (size_t)ldc * es, m * es, n, hipMemcpyHostToDevice) != hipSuccess) return kEHip;
+  rc = body(h.d[0], std::max(m, 1), h.d[1], std::max(k, 1), h.d[2], std::max(m, 1));
+  if (rc != kOK) return rc;
+  if (hipMemcpy2D(C, (size_t)ldc * es, h.d[2], m * es, m * es, n, hipMemcpyDeviceToHost) != hipSuccess) return kEHip;
+  return kOK;
+}
+
+inline void report(const char* fn, int rc) {
+  if (rc != kOK) fprintf(stderr, "%s(): error %d\n", fn, rc);
+}
+
+}  // namespace fmm
+
+extern "C" {
+
+int fmm_plan_create(fmm_plan_t* plan, int nlevels, const int* dims, int rank, const double* U, const double* V,
+                    const double* W) {
+  if (!plan) return fmm::kEInval;
+  *plan = nullptr;
+  fmm_plan_s* p = new (std::nothrow) fmm_plan_s();
+  if (!p) return fmm::kENoMem;
+  int rc = fmm::build_plan(&p->p, nlevels, dims, rank, U, V, W);
+  if (rc != fmm::kOK) {
+    delete p;
+    return rc;
+  }
+  *plan = p;
+  return fmm::kOK;
+}
+
+int fmm_plan_destroy(fmm_plan_t plan) {
+  if (!plan) return fmm::kEInval;
+  fmm::release_plan(&plan->p);
+  delete plan;
+  return fmm::kOK;
+}
+
+int fmm_plan_info(fmm_plan_t plan, int* Mx, int* Kx, int* Nx, int* rank) {
+  if (!plan) return fmm::kEInval;
+  if (Mx) *Mx = plan->p.Mx;
+  if (Kx) *Kx = plan->p.Kx;
+  if (Nx) *Nx = plan->p.Nx;
+  if (rank) *rank = plan->p.R;
+  return fmm::kOK;
+}
+
+size_t fmm_workspace_bytes(fmm_plan_t plan, int m, int n, int k) {
+  if (!plan || m <= 0 || n <= 0 || k <= 0) return 0;
+  fmm::JobList JL;
+  fmm::add_fmm_jobs(JL, plan->p, nullptr, nullptr, nullptr, nullptr, m, n, k, nullptr, std::max(m, 1), nullptr,
+                    std::max(k, 1), nullptr, std::max(m, 1));
+  return fmm::ws_ints(JL) * sizeof(int);
+}
+
+int fmm_abc(fmm_plan_t plan, int dtype, int m, int n, int k, const void* A, int lda, const void* B, int ldb, void* C,
+            int ldc, void* ws, size_t ws_bytes, void* stream) {
+  if (!plan) return fmm::kEInval;
+  if (dtype != FMM_F64) return fmm::kEUnsupported;
+  return fmm::run_abc(&plan->p, m, n, k, static_cast<const double*>(A), lda, static_cast<const double*>(B), ldb,
+                      static_cast<double*>(C), ldc, ws, ws_bytes, static_cast<hipStream_t>(stream));
+}
+
+int fmm_gemm(int dtype, int m, int n, int k, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
+             void* stream) {
+  if (dtype != FMM_F64) return fmm::kEUnsupported;
+  return fmm::run_gemm(m, n, k, static_cast<const double*>(A), lda, static_cast<const double*>(B), ldb,
+                       static_cast<double*>(C), ldc, static_cast<hipStream_t>(stream));
+}
+
+int fmm_ws_status(const void* ws, int* flags) {
+  if (!ws || !flags) return fmm::kEInval;
+  if (hipDeviceSynchronize() != hipSuccess) return fmm::kEHip;
+  int v = 0;
+  if (hipMemcpy(&v, static_cast<const int*>(ws) + 1, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    return fmm::kEHip;
+  *flags = v;
+  return v ? fmm::kESpin : fmm::kOK;
+}
+
+int fmm_host_abc(fmm_plan_t plan, int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C,
+                 int ldc) {
+  if (!plan) return fmm::kEInval;
+  return fmm::host_call(m, n, k, A, lda, B, ldb, C, ldc,
+                        [&](const double* a, int la, const double* b, int lb, double* c, int lc) {
+                          int rc = fmm::run_abc(&plan->p, m, n, k, a, la, b, lb, c, lc, nullptr, 0, nullptr);
+                          if (rc == fmm::kOK && hipDeviceSynchronize() != hipSuccess) rc = fmm::kEHip;
+                          return rc;
+                        });
+}
+
+void bl_dgemm(int m, int n, int k, double* XA, int lda, double* XB, int ldb, double* XC, int ldc) {
+  int rc = fmm::host_call(m, n, k, XA, lda, XB, ldb, XC, ldc,
+                          [&](const double* a, int la, const double* b, int lb, double* c, int lc) {
+                            int r = fmm::run_gemm(m, n, k, a, la, b, lb, c, lc, nullptr);
+                            if (r == fmm::kOK && hipDeviceSynchronize() != hipSuccess) r = fmm::kEHip;
+                            return r;
+                          });
+  fmm::report("bl_dgemm", rc);
+}
+
+void bl_dgemm_ref(int m, int n, int k, double* XA, int lda, double* XB, int ldb, double* XC, int ldc) {
+  if (m <= 0 || n <= 0 || k <= 0) return;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < m; ++i) {
+      double c = XC[(size_t)j * ldc + i];
+      for (int p = 0; p < k; ++p) c += XA[(size_t)p * lda + i] * XB[(size_t)j * ldb + p];
+      XC[(size_t)j * ldc + i] = c;
+    }
+}
+
+void bl_finalize(void) {
+  fmm::HostStage& h = fmm::host_stage();
+  std::lock_guard<std::mutex> g(h.mu);
+  h.release();
+  h.dev = -1;
+}
+
+}  // extern "C"

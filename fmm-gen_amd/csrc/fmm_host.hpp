@@ -1,0 +1,338 @@
+// fmm_host.hpp -- host side: plans, partition, job lists and launches.
+//
+// Shared by the generic runtime library (fmm_runtime.hip) and by the
+// per-job libraries the generator emits (their tables are compiled in).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "fmm_core.hpp"
+
+namespace fmm {
+
+// Status codes of the C-ABI (mirrored in include/fmm_amd.h).
+enum Status : int {
+  kOK = 0,
+  kEInval = -1,
+  kEHip = -2,
+  kENoMem = -3,
+  kEUnsupported = -4,
+  kESpin = -5,
+};
+
+// Partition granularity of the reference (common/include/bl_config.h:16-17):
+// the FMM core is md x kd x nd with md % (Mx*MR) == 0, nd % (Nx*NR) == 0.
+constexpr int kMR = 8;
+constexpr int kNR = 4;
+
+struct Plan {
+  int nlev = 0;
+  std::vector<int> dims;   // 3*nlev
+  int Mx = 1, Kx = 1, Nx = 1, R = 0;
+  int max_nnz = 0;         // largest per-product source count (A or B)
+  std::vector<Prod> prods; // R products, then Mx*Nx k-fringe products
+  std::vector<Term> terms;
+  // Device copies, one per device ordinal (created on first use there).
+  std::mutex mu;
+  std::vector<Prod*> d_prods;
+  std::vector<Term*> d_terms;
+};
+
+// Decode composed row `row` of table `which` (0: U -> A(i,p), 1: V -> B(p,j),
+// 2: W -> C(i,j)) into its block in the composed grid; digits are peeled off
+// from the last level (meta/common_mix.py:296-310).
+inline void decode_row(const std::vector<int>& dims, int which, int row, int* I, int* J) {
+  const int a = which == 1 ? 1 : 0, b = which == 0 ? 1 : 2;
+  int rb = 1, cb = 1, ii = 0, jj = 0, rest = row;
+  const int nlev = (int)dims.size() / 3;
+  for (int l = nlev - 1; l >= 0; --l) {
+    const int nr = dims[3 * l + a], nc = dims[3 * l + b];
+    const int d = rest % (nr * nc);
+    rest /= nr * nc;
+    ii += (d / nc) * rb;
+    jj += (d % nc) * cb;
+    rb *= nr;
+    cb *= nc;
+  }
+  *I = ii;
+  *J = jj;
+}
+
+// Tables are dense row-major [rows x R] in the reference's composed row order.
+inline int build_plan(Plan* P, int nlev, const int* dims, int R, const double* U, const double* V,
+                      const double* W) {
+  if (!P || nlev <= 0 || !dims || R <= 0 || !U || !V || !W) return kEInval;
+  P->nlev = nlev;
+  P->dims.assign(dims, dims + 3 * nlev);
+  P->Mx = P->Kx = P->Nx = 1;
+  for (int l = 0; l < nlev; ++l) {
+    if (dims[3 * l] <= 0 || dims[3 * l + 1] <= 0 || dims[3 * l + 2] <= 0) return kEInval;
+    P->Mx *= dims[3 * l];
+    P->Kx *= dims[3 * l + 1];
+    P->Nx *= dims[3 * l + 2];
+  }
+  if ((long long)P->Mx * P->Kx > (1 << 20) || (long long)P->Kx * P->Nx > (1 << 20) ||
+      (long long)P->Mx * P->Nx > (1 << 20))
+    return kEInval;
+  P->R = R;
+  P->prods.clear();
+  P->terms.clear();
+  P->max_nnz = 0;
+  const int rowsU = P->Mx * P->Kx, rowsV = P->Kx * P->Nx, rowsW = P->Mx * P->Nx;
+  for (int r = 0; r < R; ++r) {
+    Prod pr{};
+    auto collect = [&](const double* T, int rows, int which, int* start, int* cnt) {
+      *start = (int)P->terms.size();
+      for (int a = 0; a < rows; ++a) {
+        const double v = T[(size_t)a * R + r];
+        if (v == 0.0) continue;
+        Term t;
+        t.coef = v;
+        decode_row(P->dims, which, a, &t.bi, &t.bj);
+        P->terms.push_back(t);
+      }
+      *cnt = (int)P->terms.size() - *start;
+    };
+    collect(U, rowsU, 0, &pr.a0, &pr.na);
+    collect(V, rowsV, 1, &pr.b0, &pr.nb);
+    collect(W, rowsW, 2, &pr.c0, &pr.nc);
+    pr.fringe = 0;
+    P->max_nnz = std::max(P->max_nnz, std::max(pr.na, pr.nb));
+    P->prods.push_back(pr);
+  }
+  // k-fringe: C(I,J) += A(I, Kx) B(Kx, J) over depth kr, as one-term products
+  // ordered after all R products (bl_dgemm_util.c:325-333 "kr > 0" branch).
+  for (int I = 0; I < P->Mx; ++I)
+    for (int J = 0; J < P->Nx; ++J) {
+      Prod pr{};
+      pr.a0 = (int)P->terms.size();
+      P->terms.push_back(Term{1.0, I, P->Kx});
+      pr.na = 1;
+      pr.b0 = (int)P->terms.size();
+      P->terms.push_back(Term{1.0, P->Kx, J});
+      pr.nb = 1;
+      pr.c0 = (int)P->terms.size();
+      P->terms.push_back(Term{1.0, I, J});
+      pr.nc = 1;
+      pr.fringe = 1;
+      P->prods.push_back(pr);
+    }
+  return kOK;
+}
+
+inline int device_tables(Plan* P, Prod** dp, Term** dt) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return kEHip;
+  std::lock_guard<std::mutex> g(P->mu);
+  if ((int)P->d_prods.size() <= dev) {
+    P->d_prods.resize(dev + 1, nullptr);
+    P->d_terms.resize(dev + 1, nullptr);
+  }
+  if (!P->d_prods[dev]) {
+    Prod* a = nullptr;
+    Term* b = nullptr;
+    if (hipMalloc(&a, sizeof(Prod) * P->prods.size()) != hipSuccess) return kENoMem;
+    if (hipMalloc(&b, sizeof(Term) * P->terms.size()) != hipSuccess) {
+      (void)hipFree(a);
+      return kENoMem;
+    }
+    if (hipMemcpy(a, P->prods.data(), sizeof(Prod) * P->prods.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b, P->terms.data(), sizeof(Term) * P->terms.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(a);
+      (void)hipFree(b);
+      return kEHip;
+    }
+    P->d_prods[dev] = a;
+    P->d_terms[dev] = b;
+  }
+  *dp = P->d_prods[dev];
+  *dt = P->d_terms[dev];
+  return kOK;
+}
+
+inline void release_plan(Plan* P) {
+  for (size_t d = 0; d < P->d_prods.size(); ++d) {
+    if (P->d_prods[d] || P->d_terms[d]) {
+      int cur = 0;
+      (void)hipGetDevice(&cur);
+      (void)hipSetDevice((int)d);
+      if (P->d_prods[d]) (void)hipFree(P->d_prods[d]);
+      if (P->d_terms[d]) (void)hipFree(P->d_terms[d]);
+      (void)hipSetDevice(cur);
+    }
+  }
+  P->d_prods.clear();
+  P->d_terms.clear();
+}
+
+// The plain-GEMM plan: one product, A(0,0) B(0,0) -> C(0,0).
+inline Plan* identity_plan() {
+  static Plan* P = [] {
+    Plan* p = new Plan();
+    const int dims[3] = {1, 1, 1};
+    const double one = 1.0;
+    build_plan(p, 1, dims, 1, &one, &one, &one);
+    return p;
+  }();
+  return P;
+}
+
+// --- per-device launch properties ------------------------------------------
+struct DevInfo {
+  int cus = 0;
+  int blocks_per_cu = 0;
+  long long spin_ticks = 0;
+};
+
+template <int CH>
+inline int dev_info(DevInfo* out) {
+  static std::mutex mu;
+  static std::vector<DevInfo> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return kEHip;
+  std::lock_guard<std::mutex> g(mu);
+  if ((int)cache.size() <= dev) cache.resize(dev + 1);
+  DevInfo& d = cache[dev];
+  if (d.cus == 0) {
+    int cus = 0, khz = 0, nb = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return kEHip;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, abc_f64_kernel<CH>, kThreads, 0) != hipSuccess) return kEHip;
+    d.cus = cus;
+    d.blocks_per_cu = std::max(nb, 1);
+    d.spin_ticks = (long long)khz * 1000LL * 5LL;   // 5 s
+  }
+  *out = d;
+  return kOK;
+}
+
+// --- job construction -------------------------------------------------------
+struct JobList {
+  std::vector<Job> jobs;
+  int total_units = 0;
+  int counters = 0;       // order counters needed
+  bool ordered = false;
+};
+
+inline int tiles(int n, int t) { return (n + t - 1) / t; }
+
+inline int vec_flags(const void* A, long long lda, const void* B, long long ldb, int ks_even, const void* C,
+                     long long ldc) {
+  int f = 0;
+  if (((uintptr_t)A & 15) == 0 && (lda & 1) == 0) f |= 1;
+  if (((uintptr_t)B & 15) == 0 && (ldb & 1) == 0 && ks_even) f |= 2;
+  if (((uintptr_t)C & 15) == 0 && (ldc & 1) == 0) f |= 4;
+  return f;
+}
+
+inline void add_gemm_job(JobList& L, const Prod* dp, const Term* dt, int m, int n, int k, const double* A,
+                         long long lda, const double* B, long long ldb, double* C, long long ldc) {
+  if (m <= 0 || n <= 0 || k <= 0) return;
+  Job j{};
+  j.A = A;
+  j.B = B;
+  j.C = C;
+  j.lda = lda;
+  j.ldb = ldb;
+  j.ldc = ldc;
+  j.ms = m;
+  j.ns = n;
+  j.ks = k;
+  j.kr = 0;
+  j.tiles_m = tiles(m, BM);
+  j.tiles_n = tiles(n, BN);
+  j.nprod = 1;
+  j.unit_base = L.total_units;
+  j.order_base = -1;
+  j.flags = vec_flags(A, lda, B, ldb, 1, C, ldc);
+  j.prods = dp;
+  j.terms = dt;
+  L.total_units += j.tiles_m * j.tiles_n;
+  L.jobs.push_back(j);
+}
+
+// Jobs for one FMM call C += A*B (column-major, FP64): the two independent
+// fringe GEMMs first (they are thin and long, so they start early), then the
+// FMM core with the k-fringe products appended (bl_dynamic_peeling,
+// common/dgemm/bl_dgemm_util.c:316-353, reorganised as jobs of one launch).
+inline void add_fmm_jobs(JobList& L, const Plan& P, const Prod* dp, const Term* dt, const Prod* idp,
+                         const Term* idt, int m, int n, int k, const double* A, long long lda, const double* B,
+                         long long ldb, double* C, long long ldc) {
+  const int md = m - m % (P.Mx * kMR), kd = k - k % P.Kx, nd = n - n % (P.Nx * kNR);
+  const int ms = md / P.Mx, ks = kd / P.Kx, ns = nd / P.Nx;
+  const int kr = k - kd;
+  // (ii) right columns C[:, nd:n] += A * B[:, nd:n]
+  add_gemm_job(L, idp, idt, m, n - nd, k, A, lda, B + (long long)nd * ldb, ldb, C + (long long)nd * ldc, ldc);
+  // (iii) bottom rows C[md:m, 0:nd] += A[md:m, :] * B[:, 0:nd]
+  if (nd > 0) add_gemm_job(L, idp, idt, m - md, nd, k, A + md, lda, B, ldb, C + md, ldc);
+  if (ms <= 0 || ns <= 0) return;
+  if (ks <= 0) {
+    // no FMM core depth: the k-fringe alone is a plain GEMM on the core block
+    add_gemm_job(L, idp, idt, md, nd, k, A, lda, B, ldb, C, ldc);
+    return;
+  }
+  Job j{};
+  j.A = A;
+  j.B = B;
+  j.C = C;
+  j.lda = lda;
+  j.ldb = ldb;
+  j.ldc = ldc;
+  j.ms = ms;
+  j.ns = ns;
+  j.ks = ks;
+  j.kr = kr;
+  j.tiles_m = tiles(ms, BM);
+  j.tiles_n = tiles(ns, BN);
+  j.nprod = P.R + (kr > 0 ? P.Mx * P.Nx : 0);
+  j.unit_base = L.total_units;
+  j.order_base = j.nprod > 1 ? L.counters : -1;
+  j.flags = vec_flags(A, lda, B, ldb, (ks & 1) == 0, C, ldc);
+  j.prods = dp;
+  j.terms = dt;
+  if (j.nprod > 1) {
+    L.counters += j.tiles_m * j.tiles_n;
+    L.ordered = true;
+  }
+  L.total_units += j.nprod * j.tiles_m * j.tiles_n;
+  L.jobs.push_back(j);
+}
+
+inline size_t ws_ints(const JobList& L) { return L.ordered ? (size_t)(2 + L.counters) : 0; }
+
+// Launch a job list (<= kInlineJobs jobs, passed by value) on `stream`.
+// `ws` must hold ws_ints(L) ints when L.ordered.
+template <int CH>
+inline int launch_jobs(const JobList& JL, int* ws, hipStream_t stream) {
+  if (JL.total_units == 0) return kOK;
+  if ((int)JL.jobs.size() > kInlineJobs) return kEInval;
+  DevInfo di;
+  int rc = dev_info<CH>(&di);
+  if (rc != kOK) return rc;
+  Launch L{};
+  L.njobs = (int)JL.jobs.size();
+  L.total_units = JL.total_units;
+  L.use_ticket = JL.ordered ? 1 : 0;
+  L.jobs = nullptr;
+  L.ws = ws;
+  L.spin_limit = di.spin_ticks;
+  for (size_t i = 0; i < JL.jobs.size(); ++i) L.inl[i] = JL.jobs[i];
+  if (JL.ordered) {
+    if (!ws) return kEInval;
+    const int n = (int)ws_ints(JL);
+    hipLaunchKernelGGL(ws_clear_kernel, dim3(std::min((n + 255) / 256, 1024)), dim3(256), 0, stream, ws, n);
+    if (hipGetLastError() != hipSuccess) return kEHip;
+  }
+  const int grid = std::max(1, std::min(JL.total_units, di.cus * di.blocks_per_cu));
+  hipLaunchKernelGGL(abc_f64_kernel<CH>, dim3(grid), dim3(kThreads), 0, stream, L);
+  if (hipGetLastError() != hipSuccess) return kEHip;
+  return kOK;
+}
+
+}  // namespace fmm

@@ -1,0 +1,200 @@
+"""ctypes binding of the HIP C-ABI (include/fmm_amd.h) for torch device tensors.
+
+Matrices are column-major, as in the reference: a torch tensor ``X`` of shape
+(rows, cols) with ``X.stride(0) == 1`` and leading dimension ``X.stride(1)``.
+``colmajor()`` allocates one.  PyTorch only supplies device memory and the
+current stream here; all arithmetic runs in the library's own kernels.
+
+There is no CPU fallback: if the library has not been built, or no GPU is
+visible, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+from typing import Optional
+
+import numpy as np
+
+from .coeffs import Algorithm
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libfmm_amd.so")
+
+FMM_F64 = 0
+STATUS = {0: "FMM_OK", -1: "FMM_EINVAL", -2: "FMM_EHIP", -3: "FMM_ENOMEM",
+          -4: "FMM_EUNSUPPORTED", -5: "FMM_ESPIN"}
+
+EXPORTS = ("fmm_plan_create", "fmm_plan_destroy", "fmm_plan_info", "fmm_workspace_bytes",
+           "fmm_abc", "fmm_gemm", "fmm_ws_status", "fmm_host_abc", "bl_dgemm",
+           "bl_dgemm_ref", "bl_finalize")
+
+
+class FmmError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def _declare(L, has_job: bool = False):
+    vp, i, dp = ct.c_void_p, ct.c_int, ct.POINTER(ct.c_double)
+    L.fmm_plan_create.argtypes = [ct.POINTER(vp), i, ct.POINTER(i), i, dp, dp, dp]
+    L.fmm_plan_create.restype = i
+    L.fmm_plan_destroy.argtypes = [vp]
+    L.fmm_plan_destroy.restype = i
+    L.fmm_plan_info.argtypes = [vp] + [ct.POINTER(i)] * 4
+    L.fmm_plan_info.restype = i
+    L.fmm_workspace_bytes.argtypes = [vp, i, i, i]
+    L.fmm_workspace_bytes.restype = ct.c_size_t
+    L.fmm_abc.argtypes = [vp, i, i, i, i, vp, i, vp, i, vp, i, vp, ct.c_size_t, vp]
+    L.fmm_abc.restype = i
+    L.fmm_gemm.argtypes = [i, i, i, i, vp, i, vp, i, vp, i, vp]
+    L.fmm_gemm.restype = i
+    L.fmm_ws_status.argtypes = [vp, ct.POINTER(i)]
+    L.fmm_ws_status.restype = i
+    L.fmm_host_abc.argtypes = [vp, i, i, i, dp, i, dp, i, dp, i]
+    L.fmm_host_abc.restype = i
+    for name in ("bl_dgemm", "bl_dgemm_ref"):
+        getattr(L, name).argtypes = [i, i, i, dp, i, dp, i, dp, i]
+        getattr(L, name).restype = None
+    L.bl_finalize.argtypes = []
+    L.bl_finalize.restype = None
+    if has_job:
+        L.bl_dgemm_strassen_abc.argtypes = [i, i, i, dp, i, dp, i, dp, i]
+        L.bl_dgemm_strassen_abc.restype = None
+        L.fmm_job_abc.argtypes = [i, i, i, vp, i, vp, i, vp, i, vp, ct.c_size_t, vp]
+        L.fmm_job_abc.restype = i
+        L.fmm_job_plan.argtypes = []
+        L.fmm_job_plan.restype = vp
+    return L
+
+
+def load_library(path: str = LIB_PATH, has_job: bool = False):
+    if not os.path.exists(path):
+        raise FmmError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    return _declare(ct.CDLL(path), has_job)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = load_library()
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise FmmError(f"{what} failed: {STATUS.get(rc, rc)}")
+
+
+def _stream_ptr(stream) -> Optional[int]:
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def colmajor(rows: int, cols: int, ld: Optional[int] = None, dtype=None, device="cuda"):
+    """Uninitialised column-major (rows x cols) tensor with leading dim ld."""
+    import torch
+    ld = max(rows, 1) if ld is None else ld
+    if ld < max(rows, 1):
+        raise ValueError("ld < rows")
+    store = torch.empty((max(cols, 1), ld), dtype=dtype or torch.float64, device=device)
+    return store.t()[:rows, :cols]
+
+
+def as_colmajor(x):
+    """Column-major copy of a 2-D tensor (same device and dtype)."""
+    out = colmajor(x.shape[0], x.shape[1], dtype=x.dtype, device=x.device)
+    out.copy_(x)
+    return out
+
+
+def _ld(x) -> int:
+    if x.dim() != 2:
+        raise ValueError("expected a 2-D tensor")
+    if x.shape[0] > 1 and x.stride(0) != 1:
+        raise ValueError("tensor is not column-major (stride(0) must be 1)")
+    return max(int(x.stride(1)), x.shape[0], 1) if x.shape[1] > 1 else max(x.shape[0], 1)
+
+
+def _check_operands(A, B, C):
+    import torch
+    for name, X in (("A", A), ("B", B), ("C", C)):
+        if not X.is_cuda:
+            raise FmmError(f"{name} must be a device tensor")
+        if X.dtype != torch.float64:
+            raise FmmError(f"{name} must be float64 for this entry point")
+    m, k = A.shape
+    k2, n = B.shape
+    if k2 != k or tuple(C.shape) != (m, n):
+        raise ValueError(f"shape mismatch: A{tuple(A.shape)} B{tuple(B.shape)} C{tuple(C.shape)}")
+    return m, n, k
+
+
+class Plan:
+    """A run-time plan for one (possibly composed) algorithm."""
+
+    def __init__(self, alg: Algorithm, library=None):
+        self._lib = library or lib()
+        U, V, W = (np.ascontiguousarray(t) for t in alg.as_arrays())
+        dims = np.array([d for lv in alg.levels for d in lv], dtype=np.int32)
+        h = ct.c_void_p()
+        dp = ct.POINTER(ct.c_double)
+        check(self._lib.fmm_plan_create(ct.byref(h), len(alg.levels),
+                                        dims.ctypes.data_as(ct.POINTER(ct.c_int)), alg.rank,
+                                        U.ctypes.data_as(dp), V.ctypes.data_as(dp),
+                                        W.ctypes.data_as(dp)), "fmm_plan_create")
+        self.handle = h
+        self.alg = alg
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.fmm_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def workspace_bytes(self, m: int, n: int, k: int) -> int:
+        return int(self._lib.fmm_workspace_bytes(self.handle, m, n, k))
+
+    def abc(self, A, B, C, ws=None, stream=None) -> None:
+        """C += A @ B on the GPU by the ABC FMM (FP64, device tensors)."""
+        m, n, k = _check_operands(A, B, C)
+        wptr, wbytes = (None, 0) if ws is None else (ws.data_ptr(), ws.numel() * ws.element_size())
+        check(self._lib.fmm_abc(self.handle, FMM_F64, m, n, k, A.data_ptr(), _ld(A), B.data_ptr(),
+                                _ld(B), C.data_ptr(), _ld(C), wptr, wbytes, _stream_ptr(stream)),
+              "fmm_abc")
+
+    def host_abc(self, A: np.ndarray, B: np.ndarray, C: np.ndarray) -> None:
+        """Host-pointer drop-in on Fortran-ordered float64 numpy arrays."""
+        dp = ct.POINTER(ct.c_double)
+        m, k = A.shape
+        n = B.shape[1]
+        for X in (A, B, C):
+            if X.dtype != np.float64 or not X.flags.f_contiguous:
+                raise ValueError("host_abc needs Fortran-ordered float64 arrays")
+        check(self._lib.fmm_host_abc(self.handle, m, n, k, A.ctypes.data_as(dp), max(m, 1),
+                                     B.ctypes.data_as(dp), max(k, 1), C.ctypes.data_as(dp),
+                                     max(m, 1)), "fmm_host_abc")
+
+
+def gemm(A, B, C, stream=None, library=None) -> None:
+    """Plain C += A @ B on the same kernel (identity plan)."""
+    L = library or lib()
+    m, n, k = _check_operands(A, B, C)
+    check(L.fmm_gemm(FMM_F64, m, n, k, A.data_ptr(), _ld(A), B.data_ptr(), _ld(B), C.data_ptr(),
+                     _ld(C), _stream_ptr(stream)), "fmm_gemm")
+
+
+def ws_status(ws, library=None) -> int:
+    L = library or lib()
+    flags = ct.c_int(0)
+    L.fmm_ws_status(ws.data_ptr(), ct.byref(flags))
+    return int(flags.value)

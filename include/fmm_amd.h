@@ -1,0 +1,106 @@
+/*
+ * fmm_amd.h -- C-ABI of the MI355X (gfx950) fast-matrix-multiplication library.
+ *
+ * Two kinds of shared library export these symbols:
+ *   libfmm_amd.so      the runtime: plans built from [U,V,W] tables at run time;
+ *   lib<job>.so        one per generated job (control.py N a1 L1 ... abc),
+ *                      with that job's composed tables compiled in; it also
+ *                      exports the reference's per-job entry points below.
+ *
+ * Conventions (the reference's, common/include/bl_dgemm.h:52-130):
+ * column-major storage, C += A*B, 32-bit int dimensions and leading
+ * dimensions.  Unlike the reference, C needs no padding: nothing outside the
+ * m x n block of C is read or written.
+ */
+#ifndef FMM_AMD_H
+#define FMM_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes returned by the int-valued entry points. */
+#define FMM_OK            0
+#define FMM_EINVAL       (-1)  /* bad argument (shape, leading dimension, table) */
+#define FMM_EHIP         (-2)  /* a HIP runtime call failed                      */
+#define FMM_ENOMEM       (-3)  /* device or host allocation failed               */
+#define FMM_EUNSUPPORTED (-4)  /* dtype / variant not built                      */
+#define FMM_ESPIN        (-5)  /* a product-order wait exceeded its time bound   */
+
+/* Element types of A and B (C is FP64 for FMM_F64, FP32 otherwise). */
+#define FMM_F64  0
+
+typedef struct fmm_plan_s *fmm_plan_t;
+
+/* ---- plans ---------------------------------------------------------------
+ * A plan is one (possibly multi-level) algorithm: `nlevels` partitions
+ * dims[3*l .. 3*l+2] = (mx, kx, nx) of level l (level 0 first), and the
+ * composed tables U [Mx*Kx x rank], V [Kx*Nx x rank], W [Mx*Nx x rank],
+ * row-major, rows in the reference's composed order (level 0 = most
+ * significant digit; meta/common_mix.py:65-92, 284-310, 324-341).
+ * Replaces: meta/abc_gen_mix.py:407-431 (read_coeffs + generateCoeffs at
+ * generation time), done here at run time. */
+int fmm_plan_create(fmm_plan_t *plan, int nlevels, const int *dims, int rank,
+                    const double *U, const double *V, const double *W);
+int fmm_plan_destroy(fmm_plan_t plan);
+int fmm_plan_info(fmm_plan_t plan, int *Mx, int *Kx, int *Nx, int *rank);
+
+/* Device workspace (bytes) fmm_abc needs for this shape: a few ints of
+ * product-order counters; 0 when no ordering is needed. */
+size_t fmm_workspace_bytes(fmm_plan_t plan, int m, int n, int k);
+
+/* ---- device-pointer entries (stream ordered, reentrant) -------------------
+ * A, B, C are device pointers; `stream` is a hipStream_t (NULL = default).
+ * ws may be NULL, in which case the workspace is taken and returned with
+ * stream-ordered allocation on `stream`.  Concurrent calls must not share
+ * one ws.
+ * fmm_abc replaces: bl_dgemm_strassen_abc (meta/abc_gen_mix.py:448-472) with
+ * the peeling of common/dgemm/bl_dgemm_util.c:316-353 in the same launch. */
+int fmm_abc(fmm_plan_t plan, int dtype, int m, int n, int k,
+            const void *A, int lda, const void *B, int ldb, void *C, int ldc,
+            void *ws, size_t ws_bytes, void *stream);
+
+/* Plain GEMM C += A*B on the same kernel (identity plan).
+ * Replaces: bl_dgemm (common/dgemm/my_dgemm.c:115-210). */
+int fmm_gemm(int dtype, int m, int n, int k, const void *A, int lda,
+             const void *B, int ldb, void *C, int ldc, void *stream);
+
+/* Error flags a launch left in ws (synchronises the device first). */
+int fmm_ws_status(const void *ws, int *flags);
+
+/* ---- host-pointer drop-ins -------------------------------------------------
+ * Copy in, run on the current GPU, copy C back; return when done.
+ * fmm_host_abc replaces bl_dgemm_strassen_abc for a run-time plan. */
+int fmm_host_abc(fmm_plan_t plan, int m, int n, int k, const double *A, int lda,
+                 const double *B, int ldb, double *C, int ldc);
+
+/* The reference's plain entries, same signatures and C += A*B semantics:
+ * common/include/bl_dgemm.h:60-70 (bl_dgemm), :160-170 (bl_dgemm_ref),
+ * :207 (bl_finalize).  bl_dgemm runs on the GPU; bl_dgemm_ref is the
+ * textbook triple loop on the CPU (common/dgemm/bl_dgemm_ref.c:15-50). */
+void bl_dgemm(int m, int n, int k, double *XA, int lda, double *XB, int ldb,
+              double *XC, int ldc);
+void bl_dgemm_ref(int m, int n, int k, double *XA, int lda, double *XB,
+                  int ldb, double *XC, int ldc);
+void bl_finalize(void);
+
+/* ---- per-job libraries only (lib<job>.so from control.py) -----------------
+ * The reference's generated entries (meta/abc_gen_mix.py:448,
+ * meta/ab_gen_mix.py, meta/naive_gen_mix.py; declared in
+ * common/include/bl_dgemm.h:96-130), host pointers, the job's own algorithm: */
+void bl_dgemm_strassen_abc(int m, int n, int k, double *XA, int lda,
+                           double *XB, int ldb, double *XC, int ldc);
+/* and the device-pointer form of the same job (same contract as fmm_abc): */
+int fmm_job_abc(int m, int n, int k, const double *A, int lda, const double *B,
+                int ldb, double *C, int ldc, void *ws, size_t ws_bytes,
+                void *stream);
+/* the job's plan, for the run-time entries above */
+fmm_plan_t fmm_job_plan(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FMM_AMD_H */

@@ -1,0 +1,139 @@
+"""The CPU oracle, pinned (CPU only).
+
+* exact small-integer known-answer tests: every restated FMM variant equals
+  numpy's integer matmul bit for bit (all partial sums are exact in FP64);
+* the blocked plain GEMM and the peeling are bitwise equal to the reference's
+  own bl_dgemm / bl_dynamic_peeling compiled from its sources (oracle/_ref);
+* the multi-C epilogue C_j = C_j + alpha_j*AB equals the reference's
+  bl_dgemm_asm_8x4_mulstrassen bit for bit;
+* committed golden fixtures (tests/golden) written from the reference build;
+* FP64 error of the restated FMM stays under the stated per-level tolerance.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+import pytest
+
+from conftest import int_mats
+from tolerance import norm_err, tol_for
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+ALGS = [(["222"], [1]), (["222"], [2]), (["222", "323"], [1, 1]), (["333"], [1]),
+        (["232"], [1]), (["323"], [1]), (["grey_strassen"], [1]), (["322"], [1])]
+
+
+def aligned_f(rows, cols, align=64):
+    buf = np.zeros(rows * cols + align // 8, dtype=np.float64)
+    off = (-buf.ctypes.data % align) // 8
+    return buf[off:off + rows * cols].reshape((rows, cols), order="F")
+
+
+@pytest.mark.parametrize("spec", ALGS, ids=lambda s: "x".join(f"{a}L{l}" for a, l in zip(*s)))
+@pytest.mark.parametrize("pack", ["abc", "ab", "naive"])
+def test_kat_bit_exact(fmm, oracle, spec, pack):
+    alg = fmm.build_algorithm(*spec)
+    Mx, Kx, Nx = alg.dims
+    rng = np.random.default_rng(hash((tuple(spec[0]), pack)) & 0xFFFF)
+    shapes = [(Mx * 8 * 3, Nx * 4 * 5, Kx * 7),            # no fringe
+              (Mx * 8 * 3 + 5, Nx * 4 * 2 + 3, Kx * 5 + 1),  # every fringe
+              (Mx * 8 - 1, 17, 9),                           # no FMM core
+              (37, 29, Kx - 1 if Kx > 1 else 1)]
+    for (m, n, k) in shapes:
+        A, B, C = int_mats(rng, m, n, k)
+        want = C + A @ B
+        got = C.copy(order="F")
+        oracle.fmm(alg, A, B, got, pack)
+        assert np.array_equal(got, want), (pack, m, n, k)
+
+
+def test_blis_gemm_matches_reference_bitwise(oracle):
+    R = oracle.ref_lib()
+    if R is None:
+        pytest.skip("reference runtime not built (needs the reference sources)")
+    rng = np.random.default_rng(7)
+    for (m, n, k) in [(37, 29, 300), (128, 64, 600), (8, 4, 256), (100, 100, 1), (96, 40, 513)]:
+        A = np.asfortranarray(rng.random((m, k)))
+        B = np.asfortranarray(rng.random((k, n)))
+        ldc = ((m - 1) // 8 + 1) * 8
+        Cr = aligned_f(ldc, n + 4)
+        Co = aligned_f(ldc, n + 4)
+        Cr[:m, :n] = Co[:m, :n] = rng.random((m, n))
+        R.bl_dgemm(m, n, k, oracle._p(A), m, oracle._p(B), k, oracle._p(Cr), ldc)
+        oracle.lib().oracle_dgemm_blis(m, n, k, oracle._p(A), m, oracle._p(B), k, oracle._p(Co), ldc)
+        assert np.array_equal(Cr[:m, :n], Co[:m, :n]), (m, n, k)
+
+
+def test_peeling_matches_reference_bitwise(oracle):
+    R = oracle.ref_lib()
+    if R is None:
+        pytest.skip("reference runtime not built")
+    rng = np.random.default_rng(8)
+    for (m, n, k, d1, d2, d3) in [(70, 45, 35, 16, 2, 8), (100, 33, 300, 48, 4, 24), (20, 9, 7, 24, 3, 12)]:
+        A = np.asfortranarray(rng.random((m, k)))
+        B = np.asfortranarray(rng.random((k, n)))
+        ldc = ((m - 1) // 8 + 1) * 8
+        Cr = aligned_f(ldc, n + 4)
+        Co = aligned_f(ldc, n + 4)
+        R.bl_dynamic_peeling(m, n, k, oracle._p(A), m, oracle._p(B), k, oracle._p(Cr), ldc, d1, d2, d3)
+        oracle.lib().oracle_peeling(m, n, k, oracle._p(A), m, oracle._p(B), k, oracle._p(Co), ldc, d1, d2, d3)
+        assert np.array_equal(Cr[:m, :n], Co[:m, :n]), (m, n, k)
+
+
+def test_multi_c_epilogue_matches_reference(oracle):
+    """C_j = C_j + alpha_j * (A_panel B_panel) for 8x4 panels, the update the
+    oracle's product_update applies (bl_dgemm_asm_8x4_mulstrassen)."""
+    R = oracle.ref_lib()
+    if R is None:
+        pytest.skip("reference runtime not built")
+    rng = np.random.default_rng(9)
+    k = 37
+    packA = aligned_f(8 * k, 1)[:, 0]
+    packB = aligned_f(4 * k, 1)[:, 0]
+    packA[:] = rng.random(8 * k)
+    packB[:] = rng.random(4 * k)
+    alphas = np.array([1.0, -1.0, 0.5, -0.125], dtype=np.float64)
+    Cs = [aligned_f(8, 4) for _ in alphas]
+    for c in Cs:
+        c[:] = rng.random((8, 4))
+    want = []
+    AB = np.zeros((8, 4))
+    for p in range(k):  # sequential accumulation from 0, mul then add
+        AB = AB + np.outer(packA[p * 8:(p + 1) * 8], packB[p * 4:(p + 1) * 4])
+    for c, a in zip(Cs, alphas):
+        want.append(c + a * AB)
+    dp = ct.POINTER(ct.c_double)
+    clist = (dp * len(Cs))(*[c.ctypes.data_as(dp) for c in Cs])
+    aux = (ct.c_byte * 64)()
+    R.bl_dgemm_asm_8x4_mulstrassen(k, oracle._p(packA), oracle._p(packB), len(Cs), 8,
+                                   clist, alphas.ctypes.data_as(dp), ct.cast(aux, ct.c_void_p))
+    for c, w in zip(Cs, want):
+        assert np.array_equal(c, w)
+
+
+def test_golden_fixtures(oracle):
+    path = os.path.join(GOLDEN, "ref_dgemm.npz")
+    if not os.path.exists(path):
+        pytest.skip("golden fixtures not generated")
+    z = np.load(path, allow_pickle=False)
+    for i in range(int(z["count"])):
+        A, B, C0, Cref = (z[f"{t}{i}"] for t in ("A", "B", "C", "out"))
+        A, B = np.asfortranarray(A), np.asfortranarray(B)
+        got = np.asfortranarray(C0.copy())
+        oracle.dgemm(A, B, got)
+        assert np.array_equal(got, Cref), i
+
+
+@pytest.mark.parametrize("spec", ALGS[:6], ids=lambda s: "x".join(f"{a}L{l}" for a, l in zip(*s)))
+def test_float_error_within_stated_tolerance(fmm, oracle, spec):
+    alg = fmm.build_algorithm(*spec)
+    rng = np.random.default_rng(11)
+    m = n = 6 * 8 * 4
+    k = 384
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+    C0 = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    got = C0.copy(order="F")
+    oracle.fmm(alg, A, B, got, "abc")
+    assert norm_err(got, A, B, C0) <= tol_for(len(alg.levels))
