@@ -1,0 +1,205 @@
+"""Benchmark: effective GFLOPS (2mnk/t) of the best generated ABC FMM vs the
+project's plain FP64 MFMA GEMM, on one MI355X.
+
+One *step* is one C += A*B call (FP64, column-major, inputs resident in HBM)
+through the C-ABI entry fmm_abc.  Default workload: the one-level <3,3,3>
+rank-23 algorithm (smirnov333-23-139), square m=n=k=9600, A, B, C ~ U[-1,1)
+from a seed.  Other shapes of the same family: --workload.
+
+Printed JSON line (one, rank 0): metric/value/unit as the headline metric,
+plus the plain GEMM's rate on the same inputs, the sampled max-norm error of
+both against an extended-precision dot product, a roofline object for the
+FMM kernel and a CPU baseline (the oracle's C restatement of the reference
+ABC algorithm, timed on the host cores on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    # name: (algs, levels, m, n, k)
+    "333L1_sq9600": (["333"], [1], 9600, 9600, 9600),
+    "222x323_sq9600": (["222", "323"], [1, 1], 9600, 9600, 9600),
+    "333L1_rankk14400x480": (["333"], [1], 14400, 14400, 480),
+    "222x323_rankk14400x480": (["222", "323"], [1, 1], 14400, 14400, 480),
+    "222L1_sq2048": (["222"], [1], 2048, 2048, 2048),
+}
+DEFAULT = "333L1_sq9600"
+PEAK_F64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-gemm", action="store_true", help="skip the plain-GEMM comparison")
+    return ap.parse_args()
+
+
+def sampled_error(A, B, C0, C, nsamp=256, seed=1):
+    """Max |C - C_exact| / (u k max|A| max|B|) over sampled entries, with
+    C_exact from extended-precision dot products of the same rows/columns."""
+    import numpy as np
+    import torch
+    m, k = A.shape
+    n = B.shape[1]
+    g = np.random.default_rng(seed)
+    ii = g.integers(0, m, nsamp)
+    jj = g.integers(0, n, nsamp)
+    It = torch.as_tensor(ii, device=A.device)
+    Jt = torch.as_tensor(jj, device=A.device)
+    rows = A[It, :].cpu().numpy().astype(np.longdouble)
+    cols = B[:, Jt].cpu().numpy().astype(np.longdouble)
+    c0 = C0[It, Jt].cpu().numpy().astype(np.longdouble)
+    ex = c0 + (rows * cols.T).sum(axis=1)
+    got = C[It, Jt].cpu().numpy().astype(np.longdouble)
+    amax = float(A.abs().max())
+    bmax = float(B.abs().max())
+    return float(np.abs(got - ex).max() / (2.0 ** -53 * k * amax * bmax))
+
+
+def cpu_baseline(F, algs, levels):
+    """The oracle's C/OpenMP restatement of the reference ABC algorithm (the
+    same tables, partition, product order and packing-sum order), timed on
+    this host on a bounded sample: the same algorithm at m=n=k=3072, min of 3."""
+    import numpy as np
+    from oracle import oracle
+    alg = F.build_algorithm(algs, levels)
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except Exception:
+        ncpu = os.cpu_count() or 1
+    threads = int(os.environ.get("BLISLAB_IC_NT", min(ncpu, 16)))
+    oracle.set_threads(threads)
+    s = 3072
+    rng = np.random.default_rng(7)
+    A = np.asfortranarray(rng.uniform(-1, 1, (s, s)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (s, s)))
+    best = float("inf")
+    for _ in range(3):
+        C = np.zeros((s, s), order="F")
+        t0 = time.perf_counter()
+        oracle.fmm(alg, A, B, C, "abc")
+        best = min(best, time.perf_counter() - t0)
+    return {"value": round(2.0 * s ** 3 / best / 1e9, 2), "unit": "GFLOPS", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle C restatement of the reference ABC {'x'.join(algs)} "
+                      f"L{levels}, m=n=k={s}, FP64, min of 3"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
+              "sharding of this path is described in DESIGN.md and not built.", flush=True)
+        sys.exit(2)
+    import torch
+    from __graft_entry__ import load_package
+    F = load_package()
+    from fmm_gen_amd import runtime as rt
+
+    algs, levels, m, n, k = WORKLOADS[args.workload]
+    alg = F.build_algorithm(algs, levels)
+    plan = rt.Plan(alg)
+    gen = torch.Generator(device="cuda").manual_seed(args.seed)
+    A = rt.colmajor(m, k)
+    B = rt.colmajor(k, n)
+    C0 = rt.colmajor(m, n)
+    for X in (A, B, C0):
+        X.uniform_(-1.0, 1.0, generator=gen)
+    C = rt.colmajor(m, n)
+    ws = torch.zeros(plan.workspace_bytes(m, n, k) // 4 + 16, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    # correctness sample of one call from C0
+    C.copy_(C0)
+    plan.abc(A, B, C, ws=ws)
+    torch.cuda.synchronize()
+    err_fmm = sampled_error(A, B, C0, C)
+    assert rt.ws_status(ws) == 0
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t0 = time.perf_counter()
+        for e0, e1 in ev:
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        per = [e0.elapsed_time(e1) for e0, e1 in ev]
+        return wall, per
+
+    wall, per = timed(lambda: plan.abc(A, B, C, ws=ws), args.steps, args.warmup)
+    ms_step = wall * 1e3 / args.steps
+    eff = 2.0 * m * n * k / (ms_step * 1e-3) / 1e9
+
+    gemm = None
+    if not args.no_gemm:
+        C.copy_(C0)
+        rt.gemm(A, B, C)
+        torch.cuda.synchronize()
+        err_gemm = sampled_error(A, B, C0, C)
+        gwall, gper = timed(lambda: rt.gemm(A, B, C), args.steps, args.warmup)
+        gms = gwall * 1e3 / args.steps
+        gemm = {"gflops": round(2.0 * m * n * k / (gms * 1e-3) / 1e9, 1), "ms_per_step": round(gms, 4),
+                "kernel_ms_mean": round(sum(gper) / len(gper), 4), "max_norm_err": round(err_gemm, 3)}
+
+    # roofline of the FMM kernel (abc_f64_kernel): executed MFMA flops per
+    # launch over its mean event-timed duration
+    p = F.partition(m, n, k, alg.dims)
+    f_mfma = F.fmm_flops(alg, m, n, k)
+    kern_ms = sum(per) / len(per)
+    achieved = f_mfma / (kern_ms * 1e-3) / 1e12
+    nnzU, nnzV, nnzW = alg.nnz()
+    b_min = 8 * (nnzU * p.ms * p.ks + nnzV * p.ks * p.ns) + 8 * 2 * nnzW * p.ms * p.ns
+    roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_F64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_F64_TFLOPS, 4), "traffic": None,
+            "kernel": "abc_f64_kernel", "flops_per_launch": f_mfma,
+            "algorithmic_bytes_per_launch": b_min, "kernel_ms_mean": round(kern_ms, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(F, algs, levels)
+
+    out = {
+        "metric": "effective GFLOPS (2mnk/s) FMM vs plain GEMM at stated error",
+        "value": round(eff, 1),
+        "unit": "GFLOPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded U[-1,1) inputs, C pre-filled)",
+        "config": {"workload": args.workload, "algorithm": "x".join(f"{a}-{l}" for a, l in zip(algs, levels)),
+                   "pack_type": "abc", "m": m, "n": n, "k": k, "rank": alg.rank, "dims": list(alg.dims),
+                   "leaf": [p.ms, p.ks, p.ns], "parallelism": "single GPU"},
+        "max_norm_err": round(err_fmm, 3),
+        "err_metric": "max|C-C_exact|/(2^-53 k max|A| max|B|) on 256 sampled entries",
+        "plain_gemm": gemm,
+        "speedup_vs_gemm": round(eff / gemm["gflops"], 4) if gemm else None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

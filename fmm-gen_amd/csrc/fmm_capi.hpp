@@ -6,7 +6,7 @@
 #include <cstdlib>
 
 #include "fmm_host.hpp"
-#include "../../include/fmm_amd.h"
+#include "fmm_amd.h"
 
 #ifndef FMM_CH
 #define FMM_CH 4

@@ -233,11 +233,12 @@ __device__ __forceinline__ void store_slice(double (*As)[BM], double (*Bs)[BKP],
   for (int e = 0; e < BK / 2; ++e) Bs[g.bcol][g.bk + e] = s.b[e];
 }
 
+template <int KK0, int KK1>
 __device__ __forceinline__ void mfma_slice(const double (*As)[BM], const double (*Bs)[BKP],
                                            int lane, int wm, int wn, f64x4 (&acc)[4][4]) {
   const int l16 = lane & 15, lk = lane >> 4;
 #pragma unroll
-  for (int kk = 0; kk < BK / 4; ++kk) {
+  for (int kk = KK0; kk < KK1; ++kk) {
     double a[4], b[4];
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi) a[mi] = As[kk * 4 + lk][wm * 64 + mi * 16 + l16];
@@ -316,10 +317,57 @@ __device__ __forceinline__ void scatter_c(const Job& J, const Prod& P, int row0,
   }
 }
 
-template <int CH>
+// Contraction of one unit over its whole depth into acc.
+//   DB = false: one LDS buffer; the next slice's loads fly during the MFMAs,
+//               then sum + store + two barriers per slice.
+//   DB = true:  two LDS buffers; the next slice is summed and stored between
+//               the two halves of the current slice's MFMAs, one barrier per
+//               slice, loads issued one slice ahead.
+template <int CH, bool DB>
+__device__ __forceinline__ void contract(const Job& J, const Prod& P, const UnitGeo& g, double (*As)[BK][BM],
+                                         double (*Bs)[BN][BKP], int lane, int wm, int wn, f64x4 (&acc)[4][4]) {
+  const int nsteps = (g.depth + BK - 1) / BK;
+  if (!(P.na > 0 && P.nb > 0 && nsteps > 0)) return;
+  Raw<CH> r;
+  Slice s;
+  load_raw<CH>(J, P, P.a0, P.na, P.b0, P.nb, 0, g, r);
+  finish_sums<CH>(J, P, 0, g, r, s);
+  store_slice(As[0], Bs[0], g, s);
+  if (!DB) {
+    __syncthreads();
+    for (int st = 0; st < nsteps; ++st) {
+      const bool more = st + 1 < nsteps;
+      const int kb_next = (st + 1) * BK;
+      if (more) load_raw<CH>(J, P, P.a0, P.na, P.b0, P.nb, kb_next, g, r);
+      mfma_slice<0, BK / 4>(As[0], Bs[0], lane, wm, wn, acc);
+      __syncthreads();
+      if (more) {
+        finish_sums<CH>(J, P, kb_next, g, r, s);
+        store_slice(As[0], Bs[0], g, s);
+        __syncthreads();
+      }
+    }
+  } else {
+    if (nsteps > 1) load_raw<CH>(J, P, P.a0, P.na, P.b0, P.nb, BK, g, r);
+    __syncthreads();
+    for (int st = 0; st < nsteps; ++st) {
+      const int cur = st & 1;
+      mfma_slice<0, BK / 8>(As[cur], Bs[cur], lane, wm, wn, acc);
+      if (st + 1 < nsteps) {
+        finish_sums<CH>(J, P, (st + 1) * BK, g, r, s);
+        store_slice(As[cur ^ 1], Bs[cur ^ 1], g, s);
+        if (st + 2 < nsteps) load_raw<CH>(J, P, P.a0, P.na, P.b0, P.nb, (st + 2) * BK, g, r);
+      }
+      mfma_slice<BK / 8, BK / 4>(As[cur], Bs[cur], lane, wm, wn, acc);
+      __syncthreads();
+    }
+  }
+}
+
+template <int CH, bool DB>
 __global__ void __launch_bounds__(kThreads) abc_f64_kernel(Launch L) {
-  __shared__ double As[BK][BM];
-  __shared__ double Bs[BN][BKP];
+  __shared__ double As[DB ? 2 : 1][BK][BM];
+  __shared__ double Bs[DB ? 2 : 1][BN][BKP];
   __shared__ int s_unit;
 
   const int tid = threadIdx.x;
@@ -364,27 +412,7 @@ __global__ void __launch_bounds__(kThreads) abc_f64_kernel(Launch L) {
 #pragma unroll
       for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-    const int nsteps = (g.depth + BK - 1) / BK;
-    if (P.na > 0 && P.nb > 0 && nsteps > 0) {
-      Raw<CH> r;
-      Slice s;
-      load_raw<CH>(J, P, P.a0, P.na, P.b0, P.nb, 0, g, r);
-      finish_sums<CH>(J, P, 0, g, r, s);
-      store_slice(As, Bs, g, s);
-      __syncthreads();
-      for (int st = 0; st < nsteps; ++st) {
-        const bool more = st + 1 < nsteps;
-        const int kb_next = (st + 1) * BK;
-        if (more) load_raw<CH>(J, P, P.a0, P.na, P.b0, P.nb, kb_next, g, r);
-        mfma_slice(As, Bs, lane, wm, wn, acc);
-        __syncthreads();
-        if (more) {
-          finish_sums<CH>(J, P, kb_next, g, r, s);
-          store_slice(As, Bs, g, s);
-          __syncthreads();
-        }
-      }
-    }
+    contract<CH, DB>(J, P, g, As, Bs, lane, wm, wn, acc);
 
     const bool ordered = J.order_base >= 0;
     int* ctr = L.ws + 2 + (ordered ? J.order_base + p : 0);
