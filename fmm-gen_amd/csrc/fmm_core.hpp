@@ -14,7 +14,8 @@
 //     at a time (the reference's packA/B_add_stra_abc, meta/abc_gen_mix.py:
 //     171-251), with the next slice's loads in flight during the MFMAs;
 //   * micro-kernel: v_mfma_f64_16x16x4f64 over the whole leaf depth into a
-//     128x128 FP64 accumulator tile (4 waves x 64x64);
+//     128x128 FP64 accumulator tile (4 waves x 64x64), computed transposed
+//     (B^T A^T) so the accumulators map onto contiguous runs of C;
 //   * W scatter: C_j = C_j + w_j * acc for every C block the product feeds
 //     (meta/abc_micro_kernel_gen.py:147-196; bl_dgemm_asm_8x4_mulstrassen.c:
 //     324-383), straight from the accumulators.
@@ -248,7 +249,7 @@ __device__ __forceinline__ void mfma_slice(const double (*As)[BM], const double 
     for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
       for (int ni = 0; ni < 4; ++ni)
-        acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mi], b[ni], acc[mi][ni], 0, 0, 0);
+        acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi][ni], 0, 0, 0);
   }
 }
 
@@ -278,39 +279,36 @@ __device__ __forceinline__ void pass_turn(int* ctr, int next) {
   if (threadIdx.x == 0) __hip_atomic_store(ctr, next, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// C_j = C_j + w_j * acc for every C term of the product.  Accumulator layout of
-// v_mfma_f64_16x16x4f64: element v of lane l is row 4*(l/16)+v, column l%16 of
-// the 16x16 block.
+// C_j = C_j + w_j * acc for every C term of the product.
+//
+// v_mfma_f64_16x16x4f64 computes D[i][j] = sum_k X[i][k] Y[k][j] with X[i][k]
+// in lane i+16k, Y[k][j] in lane j+16k, and element v of lane l holding
+// D[l/16 + 4v][l%16] (layout measured on gfx950).
+// The contraction feeds X = B^T and Y = A^T, so D = (A B)^T: element v of
+// lane l is C row l%16, column l/16 + 4v of the 16x16 block.  For each v a
+// wave then touches 4 columns x 16 consecutive rows: four full 128-byte runs
+// of the column-major C.
 __device__ __forceinline__ void scatter_c(const Job& J, const Prod& P, int row0, int col0, int lane,
                                           int wm, int wn, const f64x4 (&acc)[4][4]) {
   const int l16 = lane & 15, lk = lane >> 4;
-  const bool vecC = (J.flags & 4) != 0;
   for (int t = 0; t < P.nc; ++t) {
     const Term tm = J.terms[P.c0 + t];
     const double w = tm.coef;
     double* cb = J.C + (long long)tm.bi * J.ms + (long long)tm.bj * J.ns * J.ldc;
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni) {
-      const int col = col0 + wn * 64 + ni * 16 + l16;
-      if (col >= J.ns) continue;
-      double* cc = cb + (long long)col * J.ldc;
+    for (int mi = 0; mi < 4; ++mi) {
+      const int row = row0 + wm * 64 + mi * 16 + l16;
+      if (row >= J.ms) continue;
 #pragma unroll
-      for (int mi = 0; mi < 4; ++mi) {
-        const int row = row0 + wm * 64 + mi * 16 + 4 * lk;
+      for (int ni = 0; ni < 4; ++ni) {
         const f64x4 v = acc[mi][ni];
-        if (vecC && row + 3 < J.ms) {
-          double2* p = reinterpret_cast<double2*>(cc + row);
-          double2 c0 = p[0], c1 = p[1];
-          c0.x = __builtin_fma(w, v[0], c0.x);
-          c0.y = __builtin_fma(w, v[1], c0.y);
-          c1.x = __builtin_fma(w, v[2], c1.x);
-          c1.y = __builtin_fma(w, v[3], c1.y);
-          p[0] = c0;
-          p[1] = c1;
-        } else {
 #pragma unroll
-          for (int e = 0; e < 4; ++e)
-            if (row + e < J.ms) cc[row + e] = __builtin_fma(w, v[e], cc[row + e]);
+        for (int e = 0; e < 4; ++e) {
+          const int col = col0 + wn * 64 + ni * 16 + lk + 4 * e;
+          if (col < J.ns) {
+            double* c = cb + row + (long long)col * J.ldc;
+            *c = __builtin_fma(w, v[e], *c);
+          }
         }
       }
     }
