@@ -18,8 +18,12 @@ INCLUDE = os.path.join(os.path.dirname(PKG_DIR), "include")
 ARCH = "gfx950"
 
 # -packed-fp32-ops off: keep the compiler from forming v_pk_*_f32 on its own.
+# amdgpu-mfma-vgpr-form: MFMA accumulators in VGPRs; with the default AGPR
+# form this compiler copies every accumulator AGPR<->VGPR around each loop
+# iteration of an MFMA chain, which costs more than the MFMAs themselves.
 HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-mllvm", "-amdgpu-mfma-vgpr-form=1"]
 
 
 def hipcc() -> str:

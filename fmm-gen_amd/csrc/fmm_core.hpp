@@ -14,7 +14,7 @@
 //     at a time (the reference's packA/B_add_stra_abc, meta/abc_gen_mix.py:
 //     171-251), with the next slice's loads in flight during the MFMAs;
 //   * micro-kernel: v_mfma_f64_16x16x4f64 over the whole leaf depth into a
-//     128x128 FP64 accumulator tile (4 waves x 64x64), computed transposed
+//     128x128 FP64 accumulator tile (8 waves x 64x32), computed transposed
 //     (B^T A^T) so the accumulators map onto contiguous runs of C;
 //   * W scatter: C_j = C_j + w_j * acc for every C block the product feeds
 //     (meta/abc_micro_kernel_gen.py:147-196; bl_dgemm_asm_8x4_mulstrassen.c:
@@ -63,7 +63,6 @@ struct Job {
 };
 
 constexpr int kInlineJobs = 4;
-constexpr int kThreads = 256;
 
 struct Launch {
   int njobs;
@@ -80,11 +79,65 @@ enum : int { kErrSpin = 1 };
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-// Tile geometry of the FP64 kernel.
+// Pointers reached through the job table are generic; accesses through them
+// would compile to flat_* instructions, which also count against the LDS
+// wait counter and would make every LDS wait drain the in-flight global
+// prefetch.  All job memory is device (global) memory, so cast explicitly.
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) const f64x2 gcf64x2;
+typedef __attribute__((address_space(1))) const int gcint;
+__device__ __forceinline__ gcdouble* gptr(const double* p) { return (gcdouble*)p; }
+__device__ __forceinline__ gdouble* gptr(double* p) { return (gdouble*)p; }
+__device__ __forceinline__ Term term_at(const Job& J, int i) {
+  const Term* t = J.terms + i;
+  Term r;
+  r.coef = *(gcdouble*)&t->coef;
+  r.bi = *(gcint*)&t->bi;
+  r.bj = *(gcint*)&t->bj;
+  return r;
+}
+__device__ __forceinline__ Prod prod_at(const Job& J, int q) {
+  gcint* p = (gcint*)(J.prods + q);
+  Prod r;
+  r.a0 = p[0];
+  r.na = p[1];
+  r.b0 = p[2];
+  r.nb = p[3];
+  r.c0 = p[4];
+  r.nc = p[5];
+  r.fringe = p[6];
+  r.pad = 0;
+  return r;
+}
+
+// Tile geometry of the FP64 kernel: a 128x128 C tile per workgroup of 8
+// waves (2 per SIMD), each wave owning a 64x32 sub-tile (4x2 MFMA blocks of
+// 16x16); the contraction advances BK = 16 deep per LDS slice.
+constexpr int NT = 512;
+constexpr int WAVES_M = 2;
+constexpr int WAVES_N = 4;
 constexpr int BM = 128;
 constexpr int BN = 128;
 constexpr int BK = 16;
-constexpr int BKP = BK + 1;   // padded k stride of the B tile in LDS
+constexpr int BKP = BK + 1;            // padded k stride of the B tile in LDS
+constexpr int WTM = BM / WAVES_M;      // 64 rows per wave
+constexpr int WTN = BN / WAVES_N;      // 32 columns per wave
+constexpr int FM = WTM / 16;           // 4 MFMA blocks down
+constexpr int FN = WTN / 16;           // 2 MFMA blocks across
+// A slice (BM x BK, column-major source): thread t owns rows 2*(t%64)+{0,1}
+// of columns t/64 + A_CPP*i: each wave reads one contiguous 1 KiB column run.
+constexpr int A_CPP = NT / 64;         // columns covered per pass
+constexpr int A_IT = BK / A_CPP;       // passes per slice
+constexpr int AE = 2 * A_IT;           // A values per thread per slice
+// B slice (BK x BN, column-major source): B_TPC threads per column, each
+// with B_KPT consecutive k (one 32-byte run).
+constexpr int B_TPC = NT / BN;
+constexpr int B_KPT = BK / B_TPC;
+constexpr int BE = B_KPT;              // B values per thread per slice
+static_assert(BM == 128 && A_CPP * A_IT == BK && B_TPC * B_KPT == BK, "staging layout");
+static_assert(WAVES_M * WAVES_N * 64 == NT, "wave layout");
 
 __device__ __forceinline__ const Job& job_at(const Launch& L, int j) {
   return L.jobs ? L.jobs[j] : L.inl[j];
@@ -100,23 +153,15 @@ __device__ __forceinline__ int find_job(const Launch& L, int u) {
   return lo;
 }
 
-// ---------------------------------------------------------------------------
-// Global -> register staging of one BK-deep slice for up to CH source blocks.
-//
-// A slice (BM x BK, column-major source): thread t owns rows 2*(t%64)+{0,1}
-// of columns t/64 + 4*i, i < BK/4  -> each wave reads one 1 KiB column run.
-// B slice (BK x BN, column-major source): thread t owns column t/2 and the
-// 8 consecutive k of half t%2 -> each wave reads 32 full 128-byte runs.
-// ---------------------------------------------------------------------------
 struct Slice {
-  double a[BK / 2];   // 8 values of the A sum
-  double b[BK / 2];   // 8 values of the B sum
+  double a[AE];
+  double b[BE];
 };
 
 template <int CH>
 struct Raw {
-  double a[CH][BK / 2];
-  double b[CH][BK / 2];
+  double a[CH][AE];
+  double b[CH][BE];
 };
 
 struct UnitGeo {
@@ -127,128 +172,189 @@ struct UnitGeo {
   int acol;        // this thread's first A column within the slice
   int bcol;        // this thread's B column within the tile
   int bk;          // this thread's first B k within the slice
+  bool interior;   // whole tile inside the leaf block and 16-byte accessible
+  long long aoff;  // this thread's A element offset within a block at k = 0
+  long long boff;  // this thread's B element offset within a block at k = 0
 };
 
-__device__ __forceinline__ const double* a_block(const Job& J, const Term& t) {
-  return J.A + (long long)t.bi * J.ms + (long long)t.bj * J.ks * J.lda;
+__device__ __forceinline__ gcdouble* a_block(const Job& J, const Term& t) {
+  return gptr(J.A) + (long long)t.bi * J.ms + (long long)t.bj * J.ks * J.lda;
 }
-__device__ __forceinline__ const double* b_block(const Job& J, const Term& t) {
-  return J.B + (long long)t.bi * J.ks + (long long)t.bj * J.ns * J.ldb;
+__device__ __forceinline__ gcdouble* b_block(const Job& J, const Term& t) {
+  return gptr(J.B) + (long long)t.bi * J.ks + (long long)t.bj * J.ns * J.ldb;
 }
 
+// Uniform (wave-invariant) copies of values the compiler cannot prove
+// uniform, so they live in scalar registers.
+__device__ __forceinline__ long long uni64(long long x) {
+  const int lo = __builtin_amdgcn_readfirstlane((int)(x & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(x >> 32));
+  return ((long long)hi << 32) | (unsigned int)lo;
+}
+__device__ __forceinline__ double unid(double x) { return __longlong_as_double(uni64(__double_as_longlong(x))); }
+template <class T>
+__device__ __forceinline__ T* unip(T* p) { return (T*)uni64((long long)p); }
+
+// The source blocks of one chunk of a product (up to CH terms of A and of B):
+// block base pointers and coefficients, all uniform across the workgroup.
+// A thread's element is base + its fixed offset (UnitGeo::aoff / boff).
 template <int CH>
-__device__ __forceinline__ void load_raw(const Job& J, const Prod& P, int ta0, int na,
-                                         int tb0, int nb, int kb, const UnitGeo& g,
-                                         Raw<CH>& r) {
-  const bool vecA = (J.flags & 1) != 0;
-  const bool vecB = (J.flags & 2) != 0;
-  const int row = g.row0 + g.arow;
-  const bool r0ok = row < J.ms, r1ok = row + 1 < J.ms;
+struct Srcs {
+  gcdouble* a[CH];
+  gcdouble* b[CH];
+  double u[CH];
+  double v[CH];
+  int na, nb;
+};
+
+template <int CH>
+__device__ __forceinline__ void make_srcs(const Job& J, int ta0, int na, int tb0, int nb, Srcs<CH>& S) {
+  S.na = na < CH ? na : CH;
+  S.nb = nb < CH ? nb : CH;
 #pragma unroll
   for (int t = 0; t < CH; ++t) {
-    if (t < na) {
-      const Term tm = J.terms[ta0 + t];
-      const double* src = a_block(J, tm) + row;
+    if (t < S.na) {
+      const Term tm = term_at(J, ta0 + t);
+      S.a[t] = unip(a_block(J, tm));
+      S.u[t] = unid(tm.coef);
+    } else {
+      S.a[t] = nullptr;
+      S.u[t] = 0.0;
+    }
+    if (t < S.nb) {
+      const Term tm = term_at(J, tb0 + t);
+      S.b[t] = unip(b_block(J, tm));
+      S.v[t] = unid(tm.coef);
+    } else {
+      S.b[t] = nullptr;
+      S.v[t] = 0.0;
+    }
+  }
+}
+
+// Issue the global loads of slice kb for the chunk's sources.  A full slice
+// of an interior tile takes unpredicated 16-byte loads; any other slice is
+// loaded element-wise with zero fill outside the leaf block and depth.
+template <int CH>
+__device__ __forceinline__ void load_raw(const Job& J, const Srcs<CH>& S, int kb, const UnitGeo& g, Raw<CH>& r) {
+  const bool full = g.interior && kb + BK <= g.depth;
+  if (full) {
+    const long long ka = (long long)kb * J.lda;
 #pragma unroll
-      for (int i = 0; i < BK / 4; ++i) {
-        const int col = kb + g.acol + 4 * i;
-        const bool cok = col < g.depth;
-        const double* p = src + (long long)col * J.lda;
-        if (vecA && cok && r1ok) {
-          const double2 v = *reinterpret_cast<const double2*>(p);
+    for (int t = 0; t < CH; ++t) {
+      if (t < S.na) {
+        gcdouble* base = S.a[t] + ka;
+#pragma unroll
+        for (int i = 0; i < A_IT; ++i) {
+          const f64x2 v = *(gcf64x2*)(base + (long long)(A_CPP * i) * J.lda + g.aoff);
           r.a[t][2 * i] = v.x;
           r.a[t][2 * i + 1] = v.y;
-        } else {
-          r.a[t][2 * i] = (cok && r0ok) ? p[0] : 0.0;
-          r.a[t][2 * i + 1] = (cok && r1ok) ? p[1] : 0.0;
         }
       }
-    }
-    if (t < nb) {
-      const Term tm = J.terms[tb0 + t];
-      const int col = g.col0 + g.bcol;
-      const bool cok = col < J.ns;
-      const double* src = b_block(J, tm) + (long long)col * J.ldb + kb + g.bk;
-      const int krem = g.depth - (kb + g.bk);   // valid k in this run
-      if (vecB && cok && krem >= BK / 2) {
+      if (t < S.nb) {
+        gcdouble* base = S.b[t] + kb;
 #pragma unroll
-        for (int e = 0; e < BK / 2; e += 2) {
-          const double2 v = *reinterpret_cast<const double2*>(src + e);
+        for (int e = 0; e < B_KPT; e += 2) {
+          const f64x2 v = *(gcf64x2*)(base + e + g.boff);
           r.b[t][e] = v.x;
           r.b[t][e + 1] = v.y;
         }
-      } else {
-#pragma unroll
-        for (int e = 0; e < BK / 2; ++e) r.b[t][e] = (cok && e < krem) ? src[e] : 0.0;
       }
+    }
+    return;
+  }
+  const int row = g.row0 + g.arow;
+  const bool r0ok = row < J.ms, r1ok = row + 1 < J.ms;
+  const bool bcok = g.col0 + g.bcol < J.ns;
+  const int krem = g.depth - (kb + g.bk);   // valid k in this thread's B run
+#pragma unroll
+  for (int t = 0; t < CH; ++t) {
+    if (t < S.na) {
+#pragma unroll
+      for (int i = 0; i < A_IT; ++i) {
+        const int col = kb + A_CPP * i;
+        const bool cok = g.acol + col < g.depth;
+        gcdouble* p = S.a[t] + (long long)col * J.lda + g.aoff;
+        r.a[t][2 * i] = (cok && r0ok) ? p[0] : 0.0;
+        r.a[t][2 * i + 1] = (cok && r1ok) ? p[1] : 0.0;
+      }
+    }
+    if (t < S.nb) {
+      gcdouble* p = S.b[t] + kb + g.boff;
+#pragma unroll
+      for (int e = 0; e < B_KPT; ++e) r.b[t][e] = (bcok && e < krem) ? p[e] : 0.0;
     }
   }
 }
 
 // s = u0*x0, then s = s + u_t*x_t left to right (fma == add of the exact
-// product for the dyadic table coefficients).
+// product for the dyadic table coefficients, i.e. the reference's packing
+// arithmetic, meta/abc_gen_mix.py:229-251).
 template <int CH>
-__device__ __forceinline__ void accumulate(const Job& J, int ta0, int na, int tb0, int nb,
-                                           bool first, const Raw<CH>& r, Slice& s) {
+__device__ __forceinline__ void accumulate(const Srcs<CH>& S, bool first, const Raw<CH>& r, Slice& s) {
 #pragma unroll
   for (int t = 0; t < CH; ++t) {
-    if (t < na) {
-      const double u = J.terms[ta0 + t].coef;
+    if (t < S.na) {
 #pragma unroll
-      for (int e = 0; e < BK / 2; ++e)
-        s.a[e] = (first && t == 0) ? u * r.a[t][e] : __builtin_fma(u, r.a[t][e], s.a[e]);
+      for (int e = 0; e < AE; ++e)
+        s.a[e] = (first && t == 0) ? S.u[t] * r.a[t][e] : __builtin_fma(S.u[t], r.a[t][e], s.a[e]);
     }
-    if (t < nb) {
-      const double v = J.terms[tb0 + t].coef;
+    if (t < S.nb) {
 #pragma unroll
-      for (int e = 0; e < BK / 2; ++e)
-        s.b[e] = (first && t == 0) ? v * r.b[t][e] : __builtin_fma(v, r.b[t][e], s.b[e]);
+      for (int e = 0; e < BE; ++e)
+        s.b[e] = (first && t == 0) ? S.v[t] * r.b[t][e] : __builtin_fma(S.v[t], r.b[t][e], s.b[e]);
     }
   }
 }
 
-// Build the full sums for slice kb: `r` already holds chunk 0 (issued
-// earlier); further chunks are loaded here.
+// Full operand sums of slice kb: `r` already holds chunk 0 (issued earlier);
+// products with more than CH terms load their remaining chunks here.
 template <int CH>
-__device__ __forceinline__ void finish_sums(const Job& J, const Prod& P, int kb, const UnitGeo& g,
-                                            Raw<CH>& r, Slice& s) {
-  accumulate<CH>(J, P.a0, P.na, P.b0, P.nb, true, r, s);
+__device__ __forceinline__ void finish_sums(const Job& J, const Prod& P, const Srcs<CH>& S0, int kb,
+                                            const UnitGeo& g, Raw<CH>& r, Slice& s) {
+  accumulate<CH>(S0, true, r, s);
   const int nmax = P.na > P.nb ? P.na : P.nb;
   for (int c = CH; c < nmax; c += CH) {
-    const int na = P.na - c, nb = P.nb - c;
-    load_raw<CH>(J, P, P.a0 + c, na, P.b0 + c, nb, kb, g, r);
-    accumulate<CH>(J, P.a0 + c, na, P.b0 + c, nb, false, r, s);
+    Srcs<CH> S;
+    make_srcs<CH>(J, P.a0 + c, P.na - c, P.b0 + c, P.nb - c, S);
+    load_raw<CH>(J, S, kb, g, r);
+    accumulate<CH>(S, false, r, s);
   }
 }
 
 __device__ __forceinline__ void store_slice(double (*As)[BM], double (*Bs)[BKP], const UnitGeo& g,
                                             const Slice& s) {
 #pragma unroll
-  for (int i = 0; i < BK / 4; ++i) {
+  for (int i = 0; i < A_IT; ++i) {
     double2 v;
     v.x = s.a[2 * i];
     v.y = s.a[2 * i + 1];
-    *reinterpret_cast<double2*>(&As[g.acol + 4 * i][g.arow]) = v;
+    *reinterpret_cast<double2*>(&As[g.acol + A_CPP * i][g.arow]) = v;
   }
 #pragma unroll
-  for (int e = 0; e < BK / 2; ++e) Bs[g.bcol][g.bk + e] = s.b[e];
+  for (int e = 0; e < B_KPT; ++e) Bs[g.bcol][g.bk + e] = s.b[e];
 }
 
+// MFMAs of k-steps [KK0, KK1) of the slice.  v_mfma_f64_16x16x4f64 computes
+// D[i][j] = sum_k X[i][k] Y[k][j] with X[i][k] in lane i+16k, Y[k][j] in lane
+// j+16k and element v of lane l holding D[l/16 + 4v][l%16] (layout measured
+// on gfx950).  X = B^T and Y = A^T, so D is the transposed C block: element v
+// of lane l is C row l%16, column l/16 + 4v.
 template <int KK0, int KK1>
-__device__ __forceinline__ void mfma_slice(const double (*As)[BM], const double (*Bs)[BKP],
-                                           int lane, int wm, int wn, f64x4 (&acc)[4][4]) {
+__device__ __forceinline__ void mfma_slice(const double (*As)[BM], const double (*Bs)[BKP], int lane, int wm,
+                                           int wn, f64x4 (&acc)[FM][FN]) {
   const int l16 = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int kk = KK0; kk < KK1; ++kk) {
-    double a[4], b[4];
+    double a[FM], b[FN];
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi) a[mi] = As[kk * 4 + lk][wm * 64 + mi * 16 + l16];
+    for (int mi = 0; mi < FM; ++mi) a[mi] = As[kk * 4 + lk][wm * WTM + mi * 16 + l16];
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni) b[ni] = Bs[wn * 64 + ni * 16 + l16][kk * 4 + lk];
+    for (int ni = 0; ni < FN; ++ni) b[ni] = Bs[wn * WTN + ni * 16 + l16][kk * 4 + lk];
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
+    for (int mi = 0; mi < FM; ++mi)
 #pragma unroll
-      for (int ni = 0; ni < 4; ++ni)
+      for (int ni = 0; ni < FN; ++ni)
         acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi][ni], 0, 0, 0);
   }
 }
@@ -279,34 +385,28 @@ __device__ __forceinline__ void pass_turn(int* ctr, int next) {
   if (threadIdx.x == 0) __hip_atomic_store(ctr, next, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// C_j = C_j + w_j * acc for every C term of the product.
-//
-// v_mfma_f64_16x16x4f64 computes D[i][j] = sum_k X[i][k] Y[k][j] with X[i][k]
-// in lane i+16k, Y[k][j] in lane j+16k, and element v of lane l holding
-// D[l/16 + 4v][l%16] (layout measured on gfx950).
-// The contraction feeds X = B^T and Y = A^T, so D = (A B)^T: element v of
-// lane l is C row l%16, column l/16 + 4v of the 16x16 block.  For each v a
-// wave then touches 4 columns x 16 consecutive rows: four full 128-byte runs
-// of the column-major C.
-__device__ __forceinline__ void scatter_c(const Job& J, const Prod& P, int row0, int col0, int lane,
-                                          int wm, int wn, const f64x4 (&acc)[4][4]) {
+// C_j = C_j + w_j * acc for every C term of the product, straight from the
+// accumulators: for each element v a wave touches 4 columns x 16 consecutive
+// rows, i.e. four full 128-byte runs of the column-major C.
+__device__ __forceinline__ void scatter_c(const Job& J, const Prod& P, int row0, int col0, int lane, int wm,
+                                          int wn, const f64x4 (&acc)[FM][FN]) {
   const int l16 = lane & 15, lk = lane >> 4;
   for (int t = 0; t < P.nc; ++t) {
-    const Term tm = J.terms[P.c0 + t];
+    const Term tm = term_at(J, P.c0 + t);
     const double w = tm.coef;
-    double* cb = J.C + (long long)tm.bi * J.ms + (long long)tm.bj * J.ns * J.ldc;
+    gdouble* cb = gptr(J.C) + (long long)tm.bi * J.ms + (long long)tm.bj * J.ns * J.ldc;
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi) {
-      const int row = row0 + wm * 64 + mi * 16 + l16;
+    for (int mi = 0; mi < FM; ++mi) {
+      const int row = row0 + wm * WTM + mi * 16 + l16;
       if (row >= J.ms) continue;
 #pragma unroll
-      for (int ni = 0; ni < 4; ++ni) {
+      for (int ni = 0; ni < FN; ++ni) {
         const f64x4 v = acc[mi][ni];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const int col = col0 + wn * 64 + ni * 16 + lk + 4 * e;
+          const int col = col0 + wn * WTN + ni * 16 + lk + 4 * e;
           if (col < J.ns) {
-            double* c = cb + row + (long long)col * J.ldc;
+            gdouble* c = cb + row + (long long)col * J.ldc;
             *c = __builtin_fma(w, v[e], *c);
           }
         }
@@ -315,62 +415,46 @@ __device__ __forceinline__ void scatter_c(const Job& J, const Prod& P, int row0,
   }
 }
 
-// Contraction of one unit over its whole depth into acc.
-//   DB = false: one LDS buffer; the next slice's loads fly during the MFMAs,
-//               then sum + store + two barriers per slice.
-//   DB = true:  two LDS buffers; the next slice is summed and stored between
-//               the two halves of the current slice's MFMAs, one barrier per
-//               slice, loads issued one slice ahead.
-template <int CH, bool DB>
+// Contraction of one unit over its whole depth into acc, double-buffered in
+// LDS: while the MFMAs of slice st run, slice st+1 (loaded one slice earlier)
+// is summed and stored into the other buffer and the loads of slice st+2 are
+// issued; one barrier per slice.
+template <int CH>
 __device__ __forceinline__ void contract(const Job& J, const Prod& P, const UnitGeo& g, double (*As)[BK][BM],
-                                         double (*Bs)[BN][BKP], int lane, int wm, int wn, f64x4 (&acc)[4][4]) {
+                                         double (*Bs)[BN][BKP], int lane, int wm, int wn, f64x4 (&acc)[FM][FN]) {
   const int nsteps = (g.depth + BK - 1) / BK;
   if (!(P.na > 0 && P.nb > 0 && nsteps > 0)) return;
+  Srcs<CH> S;
+  make_srcs<CH>(J, P.a0, P.na, P.b0, P.nb, S);
   Raw<CH> r;
   Slice s;
-  load_raw<CH>(J, P, P.a0, P.na, P.b0, P.nb, 0, g, r);
-  finish_sums<CH>(J, P, 0, g, r, s);
+  load_raw<CH>(J, S, 0, g, r);
+  finish_sums<CH>(J, P, S, 0, g, r, s);
   store_slice(As[0], Bs[0], g, s);
-  if (!DB) {
-    __syncthreads();
-    for (int st = 0; st < nsteps; ++st) {
-      const bool more = st + 1 < nsteps;
-      const int kb_next = (st + 1) * BK;
-      if (more) load_raw<CH>(J, P, P.a0, P.na, P.b0, P.nb, kb_next, g, r);
-      mfma_slice<0, BK / 4>(As[0], Bs[0], lane, wm, wn, acc);
-      __syncthreads();
-      if (more) {
-        finish_sums<CH>(J, P, kb_next, g, r, s);
-        store_slice(As[0], Bs[0], g, s);
-        __syncthreads();
-      }
+  if (nsteps > 1) load_raw<CH>(J, S, BK, g, r);
+  __syncthreads();
+  for (int st = 0; st < nsteps; ++st) {
+    const int cur = st & 1;
+    mfma_slice<0, BK / 8>(As[cur], Bs[cur], lane, wm, wn, acc);
+    if (st + 1 < nsteps) {
+      finish_sums<CH>(J, P, S, (st + 1) * BK, g, r, s);
+      store_slice(As[cur ^ 1], Bs[cur ^ 1], g, s);
+      if (st + 2 < nsteps) load_raw<CH>(J, S, (st + 2) * BK, g, r);
     }
-  } else {
-    if (nsteps > 1) load_raw<CH>(J, P, P.a0, P.na, P.b0, P.nb, BK, g, r);
+    mfma_slice<BK / 8, BK / 4>(As[cur], Bs[cur], lane, wm, wn, acc);
     __syncthreads();
-    for (int st = 0; st < nsteps; ++st) {
-      const int cur = st & 1;
-      mfma_slice<0, BK / 8>(As[cur], Bs[cur], lane, wm, wn, acc);
-      if (st + 1 < nsteps) {
-        finish_sums<CH>(J, P, (st + 1) * BK, g, r, s);
-        store_slice(As[cur ^ 1], Bs[cur ^ 1], g, s);
-        if (st + 2 < nsteps) load_raw<CH>(J, P, P.a0, P.na, P.b0, P.nb, (st + 2) * BK, g, r);
-      }
-      mfma_slice<BK / 8, BK / 4>(As[cur], Bs[cur], lane, wm, wn, acc);
-      __syncthreads();
-    }
   }
 }
 
-template <int CH, bool DB>
-__global__ void __launch_bounds__(kThreads) abc_f64_kernel(Launch L) {
-  __shared__ double As[DB ? 2 : 1][BK][BM];
-  __shared__ double Bs[DB ? 2 : 1][BN][BKP];
+template <int CH>
+__global__ void __launch_bounds__(NT) abc_f64_kernel(Launch L) {
+  __shared__ double As[2][BK][BM];
+  __shared__ double Bs[2][BN][BKP];
   __shared__ int s_unit;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % WAVES_M, wn = wave / WAVES_M;
 
   int iter = 0;
   while (true) {
@@ -393,24 +477,34 @@ __global__ void __launch_bounds__(kThreads) abc_f64_kernel(Launch L) {
     const int p = ul - q * ntile;
     const int tmi = p / J.tiles_n;
     const int tni = p - tmi * J.tiles_n;
-    const Prod P = J.prods[q];
+    Prod P = prod_at(J, q);
+    P.a0 = __builtin_amdgcn_readfirstlane(P.a0);
+    P.na = __builtin_amdgcn_readfirstlane(P.na);
+    P.b0 = __builtin_amdgcn_readfirstlane(P.b0);
+    P.nb = __builtin_amdgcn_readfirstlane(P.nb);
+    P.c0 = __builtin_amdgcn_readfirstlane(P.c0);
+    P.nc = __builtin_amdgcn_readfirstlane(P.nc);
+    P.fringe = __builtin_amdgcn_readfirstlane(P.fringe);
 
     UnitGeo g;
     g.row0 = tmi * BM;
     g.col0 = tni * BN;
     g.depth = P.fringe ? J.kr : J.ks;
+    g.interior = (J.flags & 3) == 3 && g.row0 + BM <= J.ms && g.col0 + BN <= J.ns;
+    g.aoff = g.row0 + g.arow + (long long)g.acol * J.lda;
+    g.boff = (long long)(g.col0 + g.bcol) * J.ldb + g.bk;
     g.arow = 2 * (tid & 63);
     g.acol = tid >> 6;
-    g.bcol = tid >> 1;
-    g.bk = (BK / 2) * (tid & 1);
+    g.bcol = tid / B_TPC;
+    g.bk = B_KPT * (tid % B_TPC);
 
-    f64x4 acc[4][4];
+    f64x4 acc[FM][FN];
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
+    for (int mi = 0; mi < FM; ++mi)
 #pragma unroll
-      for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = f64x4{0.0, 0.0, 0.0, 0.0};
+      for (int ni = 0; ni < FN; ++ni) acc[mi][ni] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-    contract<CH, DB>(J, P, g, As, Bs, lane, wm, wn, acc);
+    contract<CH>(J, P, g, As, Bs, lane, wm, wn, acc);
 
     const bool ordered = J.order_base >= 0;
     int* ctr = L.ws + 2 + (ordered ? J.order_base + p : 0);

@@ -191,7 +191,7 @@ struct DevInfo {
   long long spin_ticks = 0;
 };
 
-template <int CH, bool DB>
+template <int CH>
 inline int dev_info(DevInfo* out) {
   static std::mutex mu;
   static std::vector<DevInfo> cache;
@@ -204,7 +204,7 @@ inline int dev_info(DevInfo* out) {
     int cus = 0, khz = 0, nb = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return kEHip;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, abc_f64_kernel<CH, DB>, kThreads, 0) != hipSuccess) return kEHip;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, abc_f64_kernel<CH>, NT, 0) != hipSuccess) return kEHip;
     d.cus = cus;
     d.blocks_per_cu = std::max(nb, 1);
     d.spin_ticks = (long long)khz * 1000LL * 5LL;   // 5 s
@@ -309,12 +309,12 @@ inline size_t ws_ints(const JobList& L) { return L.ordered ? (size_t)(2 + L.coun
 
 // Launch a job list (<= kInlineJobs jobs, passed by value) on `stream`.
 // `ws` must hold ws_ints(L) ints when L.ordered.
-template <int CH, bool DB>
-inline int launch_jobs_v(const JobList& JL, int* ws, hipStream_t stream) {
+template <int CH>
+inline int launch_jobs(const JobList& JL, int* ws, hipStream_t stream) {
   if (JL.total_units == 0) return kOK;
   if ((int)JL.jobs.size() > kInlineJobs) return kEInval;
   DevInfo di;
-  int rc = dev_info<CH, DB>(&di);
+  int rc = dev_info<CH>(&di);
   if (rc != kOK) return rc;
   Launch L{};
   L.njobs = (int)JL.jobs.size();
@@ -331,24 +331,9 @@ inline int launch_jobs_v(const JobList& JL, int* ws, hipStream_t stream) {
     if (hipGetLastError() != hipSuccess) return kEHip;
   }
   const int grid = std::max(1, std::min(JL.total_units, di.cus * di.blocks_per_cu));
-  abc_f64_kernel<CH, DB><<<dim3(grid), dim3(kThreads), 0, stream>>>(L);
+  abc_f64_kernel<CH><<<dim3(grid), dim3(NT), 0, stream>>>(L);
   if (hipGetLastError() != hipSuccess) return kEHip;
   return kOK;
-}
-
-// Kernel variant: double-buffered LDS unless FMM_SINGLE_BUFFER=1 (kept for
-// A/B timing of the two pipelines).
-inline bool use_double_buffer() {
-  static const bool db = [] {
-    const char* e = getenv("FMM_SINGLE_BUFFER");
-    return !(e && e[0] == '1');
-  }();
-  return db;
-}
-
-template <int CH>
-inline int launch_jobs(const JobList& JL, int* ws, hipStream_t stream) {
-  return use_double_buffer() ? launch_jobs_v<CH, true>(JL, ws, stream) : launch_jobs_v<CH, false>(JL, ws, stream);
 }
 
 }  // namespace fmm
