@@ -93,7 +93,7 @@ def emit_source(alg: Algorithm, job: str, pack_type: str) -> str:
     if pack_type != "abc":
         raise ValueError(f"pack_type {pack_type!r}: only 'abc' is generated as HIP")
     Mx, Kx, Nx = alg.dims
-    ch = min(max_terms(alg), 8)
+    ch = max_terms(alg)
     dims = ", ".join(str(d) for lv in alg.levels for d in lv)
     eq = "\n".join("// " + e for e in equations(alg))
     src = f"""// Generated by fmm-gen_amd control.py for job {job}: do not edit.
@@ -103,9 +103,8 @@ def emit_source(alg: Algorithm, job: str, pack_type: str) -> str:
 // (row-block, column-block) of the composed grid):
 {eq}
 //
-// FMM_CH = largest number of A or B terms of one product: every product's
-// operand sums are loaded in a single chunk.
-#define FMM_CH {ch}
+// Largest number of A or B terms of one product: {ch} (the kernel variant is
+// chosen from it: 32-deep slices for at most 2 terms, else 16-deep).
 #include "fmm_capi.hpp"
 
 namespace {{

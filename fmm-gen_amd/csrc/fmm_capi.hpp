@@ -8,10 +8,6 @@
 #include "fmm_host.hpp"
 #include "fmm_amd.h"
 
-#ifndef FMM_CH
-#define FMM_CH 4
-#endif
-
 struct fmm_plan_s {
   fmm::Plan p;
 };
@@ -47,7 +43,7 @@ inline int run_abc(Plan* P, int m, int n, int k, const double* A, int lda, const
   } else if (need > ws_bytes && w) {
     return kEInval;
   }
-  rc = launch_jobs<FMM_CH>(JL, w, s);
+  rc = launch_jobs(JL, P->max_nnz, w, s);
   if (own) (void)hipFreeAsync(w, s);
   return rc;
 }
@@ -62,7 +58,7 @@ inline int run_gemm(int m, int n, int k, const double* A, int lda, const double*
   if (rc != kOK) return rc;
   JobList JL;
   add_gemm_job(JL, ip, it, m, n, k, A, lda, B, ldb, C, ldc);
-  return launch_jobs<FMM_CH>(JL, nullptr, s);
+  return launch_jobs(JL, 1, nullptr, s);
 }
 
 // Staging buffers of the host-pointer drop-ins (grow-only, one set per
@@ -191,7 +187,7 @@ int fmm_ws_status(const void* ws, int* flags) {
   if (!ws || !flags) return fmm::kEInval;
   if (hipDeviceSynchronize() != hipSuccess) return fmm::kEHip;
   int v = 0;
-  if (hipMemcpy(&v, static_cast<const int*>(ws) + 1, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+  if (hipMemcpy(&v, static_cast<const int*>(ws) + fmm::kWsError, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
     return fmm::kEHip;
   *flags = v;
   return v ? fmm::kESpin : fmm::kOK;

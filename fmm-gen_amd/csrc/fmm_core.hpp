@@ -32,6 +32,16 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#ifndef FMM_PINGPONG
+#define FMM_PINGPONG 1
+#endif
+#ifndef FMM_SETPRIO
+#define FMM_SETPRIO 1
+#endif
+#ifndef FMM_FRAG_PIPE
+#define FMM_FRAG_PIPE 1
+#endif
+
 namespace fmm {
 
 // One coefficient of a product: block (bi, bj) of the operand, scaled by coef.
@@ -55,8 +65,10 @@ struct Job {
   int ms, ns, ks, kr;       // leaf block and k-fringe depth
   int tiles_m, tiles_n;     // C tiles over one leaf block
   int nprod;                // products run by this job (R [+ Mx*Nx fringe])
+  int gsize;                // products per unit (consecutive, same C tile)
+  int ngroups;              // ceil(nprod / gsize)
   int unit_base;            // first unit of this job in the launch
-  int order_base;           // first order counter, <0 when nprod == 1
+  int order_base;           // first order counter, <0 when ngroups == 1
   int flags;                // bit0/1/2: A/B/C allow 16-byte vector access
   const Prod* prods;
   const Term* terms;
@@ -67,15 +79,23 @@ constexpr int kInlineJobs = 4;
 struct Launch {
   int njobs;
   int total_units;
-  int use_ticket;           // 1: ws[0] is the unit ticket (ordered launches)
-  int pad;
+  int use_ticket;           // 1: units are claimed from per-queue tickets in ws
+  int nq;                   // unit queues (1 or 8, one per XCD)
   const Job* jobs;          // device job table, or nullptr -> inl[]
-  int* ws;                  // [0] ticket, [1] error flags, [2..] order counters
+  int* ws;                  // workspace, layout kWs* below
   long long spin_limit;     // bounded wait, in wall_clock64() ticks
   Job inl[kInlineJobs];
 };
 
 enum : int { kErrSpin = 1 };
+
+// Workspace layout (ints): queue x's ticket at kWsTicket*x (one 128-byte line
+// each), the error flags at kWsError, the per-tile order counters from
+// kWsCounters on.
+constexpr int kMaxQueues = 8;
+constexpr int kWsTicket = 32;
+constexpr int kWsError = kWsTicket * kMaxQueues;
+constexpr int kWsCounters = kWsError + 8;
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -114,14 +134,12 @@ __device__ __forceinline__ Prod prod_at(const Job& J, int q) {
 
 // Tile geometry of the FP64 kernel: a 128x128 C tile per workgroup of 8
 // waves (2 per SIMD), each wave owning a 64x32 sub-tile (4x2 MFMA blocks of
-// 16x16); the contraction advances BK = 16 deep per LDS slice.
+// 16x16); the contraction advances BK deep per LDS slice.
 constexpr int NT = 512;
 constexpr int WAVES_M = 2;
 constexpr int WAVES_N = 4;
 constexpr int BM = 128;
 constexpr int BN = 128;
-constexpr int BK = 16;
-constexpr int BKP = BK + 1;            // padded k stride of the B tile in LDS
 constexpr int WTM = BM / WAVES_M;      // 64 rows per wave
 constexpr int WTN = BN / WAVES_N;      // 32 columns per wave
 constexpr int FM = WTM / 16;           // 4 MFMA blocks down
@@ -129,14 +147,23 @@ constexpr int FN = WTN / 16;           // 2 MFMA blocks across
 // A slice (BM x BK, column-major source): thread t owns rows 2*(t%64)+{0,1}
 // of columns t/64 + A_CPP*i: each wave reads one contiguous 1 KiB column run.
 constexpr int A_CPP = NT / 64;         // columns covered per pass
-constexpr int A_IT = BK / A_CPP;       // passes per slice
-constexpr int AE = 2 * A_IT;           // A values per thread per slice
 // B slice (BK x BN, column-major source): B_TPC threads per column, each
-// with B_KPT consecutive k (one 32-byte run).
+// with BK/B_TPC consecutive k.
 constexpr int B_TPC = NT / BN;
-constexpr int B_KPT = BK / B_TPC;
-constexpr int BE = B_KPT;              // B values per thread per slice
-static_assert(BM == 128 && A_CPP * A_IT == BK && B_TPC * B_KPT == BK, "staging layout");
+
+// Slice depth BK is a template parameter: 32 suits products whose operand
+// sums have at most 2 terms (fewer barriers per flop), 16 leaves registers
+// for 4 terms per chunk.
+template <int BK_>
+struct TileK {
+  static constexpr int BK = BK_;
+  static constexpr int BKP = BK + 1;          // padded k stride of the B tile in LDS
+  static constexpr int A_IT = BK / A_CPP;     // A passes per slice
+  static constexpr int AE = 2 * A_IT;         // A values per thread per slice
+  static constexpr int B_KPT = BK / B_TPC;    // B k-run per thread
+  static constexpr int BE = B_KPT;            // B values per thread per slice
+  static_assert(BM == 128 && A_CPP * A_IT == BK && B_TPC * B_KPT == BK, "staging layout");
+};
 static_assert(WAVES_M * WAVES_N * 64 == NT, "wave layout");
 
 __device__ __forceinline__ const Job& job_at(const Launch& L, int j) {
@@ -153,15 +180,16 @@ __device__ __forceinline__ int find_job(const Launch& L, int u) {
   return lo;
 }
 
+template <int BK>
 struct Slice {
-  double a[AE];
-  double b[BE];
+  double a[TileK<BK>::AE];
+  double b[TileK<BK>::BE];
 };
 
-template <int CH>
+template <int BK, int CH>
 struct Raw {
-  double a[CH][AE];
-  double b[CH][BE];
+  double a[CH][TileK<BK>::AE];
+  double b[CH][TileK<BK>::BE];
 };
 
 struct UnitGeo {
@@ -235,8 +263,10 @@ __device__ __forceinline__ void make_srcs(const Job& J, int ta0, int na, int tb0
 // Issue the global loads of slice kb for the chunk's sources.  A full slice
 // of an interior tile takes unpredicated 16-byte loads; any other slice is
 // loaded element-wise with zero fill outside the leaf block and depth.
-template <int CH>
-__device__ __forceinline__ void load_raw(const Job& J, const Srcs<CH>& S, int kb, const UnitGeo& g, Raw<CH>& r) {
+template <int BK, int CH>
+__device__ __forceinline__ void load_raw(const Job& J, const Srcs<CH>& S, int kb, const UnitGeo& g,
+                                         Raw<BK, CH>& r) {
+  constexpr int A_IT = TileK<BK>::A_IT, B_KPT = TileK<BK>::B_KPT;
   const bool full = g.interior && kb + BK <= g.depth;
   if (full) {
     const long long ka = (long long)kb * J.lda;
@@ -290,8 +320,9 @@ __device__ __forceinline__ void load_raw(const Job& J, const Srcs<CH>& S, int kb
 // s = u0*x0, then s = s + u_t*x_t left to right (fma == add of the exact
 // product for the dyadic table coefficients, i.e. the reference's packing
 // arithmetic, meta/abc_gen_mix.py:229-251).
-template <int CH>
-__device__ __forceinline__ void accumulate(const Srcs<CH>& S, bool first, const Raw<CH>& r, Slice& s) {
+template <int BK, int CH>
+__device__ __forceinline__ void accumulate(const Srcs<CH>& S, bool first, const Raw<BK, CH>& r, Slice<BK>& s) {
+  constexpr int AE = TileK<BK>::AE, BE = TileK<BK>::BE;
 #pragma unroll
   for (int t = 0; t < CH; ++t) {
     if (t < S.na) {
@@ -309,21 +340,23 @@ __device__ __forceinline__ void accumulate(const Srcs<CH>& S, bool first, const 
 
 // Full operand sums of slice kb: `r` already holds chunk 0 (issued earlier);
 // products with more than CH terms load their remaining chunks here.
-template <int CH>
+template <int BK, int CH>
 __device__ __forceinline__ void finish_sums(const Job& J, const Prod& P, const Srcs<CH>& S0, int kb,
-                                            const UnitGeo& g, Raw<CH>& r, Slice& s) {
-  accumulate<CH>(S0, true, r, s);
+                                            const UnitGeo& g, Raw<BK, CH>& r, Slice<BK>& s) {
+  accumulate<BK, CH>(S0, true, r, s);
   const int nmax = P.na > P.nb ? P.na : P.nb;
   for (int c = CH; c < nmax; c += CH) {
     Srcs<CH> S;
     make_srcs<CH>(J, P.a0 + c, P.na - c, P.b0 + c, P.nb - c, S);
-    load_raw<CH>(J, S, kb, g, r);
-    accumulate<CH>(S, false, r, s);
+    load_raw<BK, CH>(J, S, kb, g, r);
+    accumulate<BK, CH>(S, false, r, s);
   }
 }
 
-__device__ __forceinline__ void store_slice(double (*As)[BM], double (*Bs)[BKP], const UnitGeo& g,
-                                            const Slice& s) {
+template <int BK>
+__device__ __forceinline__ void store_slice(double (*As)[BM], double (*Bs)[TileK<BK>::BKP], const UnitGeo& g,
+                                            const Slice<BK>& s) {
+  constexpr int A_IT = TileK<BK>::A_IT, B_KPT = TileK<BK>::B_KPT;
 #pragma unroll
   for (int i = 0; i < A_IT; ++i) {
     double2 v;
@@ -340,10 +373,45 @@ __device__ __forceinline__ void store_slice(double (*As)[BM], double (*Bs)[BKP],
 // j+16k and element v of lane l holding D[l/16 + 4v][l%16] (layout measured
 // on gfx950).  X = B^T and Y = A^T, so D is the transposed C block: element v
 // of lane l is C row l%16, column l/16 + 4v.
-template <int KK0, int KK1>
-__device__ __forceinline__ void mfma_slice(const double (*As)[BM], const double (*Bs)[BKP], int lane, int wm,
+template <int BK, int KK0, int KK1>
+__device__ __forceinline__ void mfma_slice(const double (*As)[BM], const double (*Bs)[TileK<BK>::BKP], int lane, int wm,
                                            int wn, f64x4 (&acc)[FM][FN]) {
   const int l16 = lane & 15, lk = lane >> 4;
+#if FMM_SETPRIO
+  __builtin_amdgcn_s_setprio(1);
+#endif
+#if FMM_FRAG_PIPE
+  // Fragments of step kk+1 are read while the MFMAs of step kk issue.
+  double a[2][FM], b[2][FN];
+#pragma unroll
+  for (int mi = 0; mi < FM; ++mi) a[0][mi] = As[KK0 * 4 + lk][wm * WTM + mi * 16 + l16];
+#pragma unroll
+  for (int ni = 0; ni < FN; ++ni) b[0][ni] = Bs[wn * WTN + ni * 16 + l16][KK0 * 4 + lk];
+#pragma unroll
+  for (int kk = KK0; kk < KK1; ++kk) {
+    const int c = (kk - KK0) & 1;
+    if (kk + 1 < KK1) {
+#pragma unroll
+      for (int mi = 0; mi < FM; ++mi) a[c ^ 1][mi] = As[(kk + 1) * 4 + lk][wm * WTM + mi * 16 + l16];
+#pragma unroll
+      for (int ni = 0; ni < FN; ++ni) b[c ^ 1][ni] = Bs[wn * WTN + ni * 16 + l16][(kk + 1) * 4 + lk];
+    }
+#pragma unroll
+    for (int mi = 0; mi < FM; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < FN; ++ni)
+        acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[c][ni], a[c][mi], acc[mi][ni], 0, 0, 0);
+    if (kk + 1 < KK1) {
+      // interleave: one LDS read after every two MFMAs
+#pragma unroll
+      for (int x = 0; x < (FM + FN) / 2; ++x) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, FM * FN, 0);
+    }
+  }
+#else
 #pragma unroll
   for (int kk = KK0; kk < KK1; ++kk) {
     double a[FM], b[FN];
@@ -357,21 +425,29 @@ __device__ __forceinline__ void mfma_slice(const double (*As)[BM], const double 
       for (int ni = 0; ni < FN; ++ni)
         acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi][ni], 0, 0, 0);
   }
+#endif
+#if FMM_SETPRIO
+  __builtin_amdgcn_s_setprio(0);
+#endif
 }
 
-// Wait (thread 0, bounded) until counter reaches `want`; then every thread
-// acquires at agent scope so C reads see the previous product's writes.
+// Wait (thread 0, bounded) until counter reaches `want`, polling with relaxed
+// loads (an acquiring poll would invalidate this XCD's L2 on every
+// iteration); then every thread acquires once at agent scope so its C reads
+// see the previous product's writes.
 __device__ __forceinline__ void wait_turn(int* ctr, int want, const Launch& L) {
   if (threadIdx.x == 0) {
-    const long long t0 = wall_clock64();
-    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
-      __builtin_amdgcn_s_sleep(4);
-      // A failed wait anywhere in the launch ends every later wait at once,
-      // so a broken ordering costs one time bound, not one per unit.
-      if (__hip_atomic_load(L.ws + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-      if (wall_clock64() - t0 > L.spin_limit) {
-        atomicOr(L.ws + 1, (int)kErrSpin);
-        break;
+    if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
+      const long long t0 = wall_clock64();
+      while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
+        __builtin_amdgcn_s_sleep(4);
+        // A failed wait anywhere in the launch ends every later wait at once,
+        // so a broken ordering costs one time bound, not one per unit.
+        if (__hip_atomic_load(L.ws + kWsError, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        if (wall_clock64() - t0 > L.spin_limit) {
+          atomicOr(L.ws + kWsError, (int)kErrSpin);
+          break;
+        }
       }
     }
   }
@@ -419,97 +495,172 @@ __device__ __forceinline__ void scatter_c(const Job& J, const Prod& P, int row0,
 // LDS: while the MFMAs of slice st run, slice st+1 (loaded one slice earlier)
 // is summed and stored into the other buffer and the loads of slice st+2 are
 // issued; one barrier per slice.
-template <int CH>
+// One slice of the pipelined contraction: MFMAs on LDS buffer `cur` while
+// slice st+1 (loaded one slice earlier) is summed into the other buffer and
+// the loads of slice st+2 are issued.  The two waves that share a SIMD run
+// the phases in opposite order (PINGPONG): wave group 0 does MFMA half 1,
+// sums, MFMA half 2; group 1 does the sums first, so one wave's VALU work and
+// load waits overlap the other wave's MFMAs instead of idling the matrix pipe.
+template <int BK, int CH>
+__device__ __forceinline__ void sums_phase(const Job& J, const Prod& P, const Srcs<CH>& S, int st, int nsteps,
+                                           const UnitGeo& g, Raw<BK, CH>& r, Slice<BK>& s, double (*As)[BK][BM],
+                                           double (*Bs)[BN][TileK<BK>::BKP], int nxt) {
+  if (st + 1 < nsteps) {
+    finish_sums<BK, CH>(J, P, S, (st + 1) * BK, g, r, s);
+    store_slice<BK>(As[nxt], Bs[nxt], g, s);
+    if (st + 2 < nsteps) load_raw<BK, CH>(J, S, (st + 2) * BK, g, r);
+  }
+}
+
+template <int BK, int CH>
 __device__ __forceinline__ void contract(const Job& J, const Prod& P, const UnitGeo& g, double (*As)[BK][BM],
-                                         double (*Bs)[BN][BKP], int lane, int wm, int wn, f64x4 (&acc)[FM][FN]) {
+                                         double (*Bs)[BN][TileK<BK>::BKP], int lane, int wm, int wn, int grp,
+                                         f64x4 (&acc)[FM][FN]) {
   const int nsteps = (g.depth + BK - 1) / BK;
   if (!(P.na > 0 && P.nb > 0 && nsteps > 0)) return;
   Srcs<CH> S;
   make_srcs<CH>(J, P.a0, P.na, P.b0, P.nb, S);
-  Raw<CH> r;
-  Slice s;
-  load_raw<CH>(J, S, 0, g, r);
-  finish_sums<CH>(J, P, S, 0, g, r, s);
-  store_slice(As[0], Bs[0], g, s);
-  if (nsteps > 1) load_raw<CH>(J, S, BK, g, r);
+  Raw<BK, CH> r;
+  Slice<BK> s;
+  load_raw<BK, CH>(J, S, 0, g, r);
+  finish_sums<BK, CH>(J, P, S, 0, g, r, s);
+  store_slice<BK>(As[0], Bs[0], g, s);
+  if (nsteps > 1) load_raw<BK, CH>(J, S, BK, g, r);
   __syncthreads();
   for (int st = 0; st < nsteps; ++st) {
     const int cur = st & 1;
-    mfma_slice<0, BK / 8>(As[cur], Bs[cur], lane, wm, wn, acc);
-    if (st + 1 < nsteps) {
-      finish_sums<CH>(J, P, S, (st + 1) * BK, g, r, s);
-      store_slice(As[cur ^ 1], Bs[cur ^ 1], g, s);
-      if (st + 2 < nsteps) load_raw<CH>(J, S, (st + 2) * BK, g, r);
+#if FMM_PINGPONG
+    if (grp == 0) {
+      mfma_slice<BK, 0, BK / 8>(As[cur], Bs[cur], lane, wm, wn, acc);
+      sums_phase<BK, CH>(J, P, S, st, nsteps, g, r, s, As, Bs, cur ^ 1);
+      mfma_slice<BK, BK / 8, BK / 4>(As[cur], Bs[cur], lane, wm, wn, acc);
+    } else {
+      sums_phase<BK, CH>(J, P, S, st, nsteps, g, r, s, As, Bs, cur ^ 1);
+      mfma_slice<BK, 0, BK / 4>(As[cur], Bs[cur], lane, wm, wn, acc);
     }
-    mfma_slice<BK / 8, BK / 4>(As[cur], Bs[cur], lane, wm, wn, acc);
+#else
+    (void)grp;
+    mfma_slice<BK, 0, BK / 8>(As[cur], Bs[cur], lane, wm, wn, acc);
+    sums_phase<BK, CH>(J, P, S, st, nsteps, g, r, s, As, Bs, cur ^ 1);
+    mfma_slice<BK, BK / 8, BK / 4>(As[cur], Bs[cur], lane, wm, wn, acc);
+#endif
     __syncthreads();
   }
 }
 
-template <int CH>
+// Map queue-local unit index i of queue x to (job, product group, tile).
+// Each job's tiles are split into nq contiguous chunks; queue x runs, job by
+// job, all product groups over its chunk (groups in order, tiles innermost),
+// so every unit a unit waits for (same tile, earlier group) sits earlier in
+// the same queue.  Jobs in a device table (many small problems) use one queue.
+__device__ __forceinline__ bool decode_unit(const Launch& L, int x, int i, int* jo, int* qo, int* po) {
+  if (L.jobs) {
+    if (i >= L.total_units) return false;
+    const int j = find_job(L, i);
+    const Job& J = L.jobs[j];
+    const int ntile = J.tiles_m * J.tiles_n;
+    const int ul = i - J.unit_base;
+    *jo = j;
+    *qo = ul / ntile;
+    *po = ul - *qo * ntile;
+    return true;
+  }
+  for (int j = 0; j < L.njobs; ++j) {
+    const Job& J = L.inl[j];
+    const int ntile = J.tiles_m * J.tiles_n;
+    const int c0 = (int)(((long long)x * ntile) / L.nq);
+    const int c1 = (int)(((long long)(x + 1) * ntile) / L.nq);
+    const int cs = c1 - c0;
+    const int n = cs * J.ngroups;
+    if (i < n) {
+      *jo = j;
+      *qo = i / cs;
+      *po = c0 + (i - *qo * cs);
+      return true;
+    }
+    i -= n;
+  }
+  return false;
+}
+
+template <int BK, int CH>
 __global__ void __launch_bounds__(NT) abc_f64_kernel(Launch L) {
+  constexpr int B_KPT = TileK<BK>::B_KPT;
   __shared__ double As[2][BK][BM];
-  __shared__ double Bs[2][BN][BKP];
+  __shared__ double Bs[2][BN][TileK<BK>::BKP];
   __shared__ int s_unit;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+  // Queue of this workgroup: workgroups are dispatched to the 8 XCDs round
+  // robin, so blockIdx % 8 keeps a queue's tiles (and their L2 reuse) on one
+  // XCD.  Correctness does not depend on the placement.
+  const int x = (int)(blockIdx.x % (unsigned)L.nq);
+  const int wq = (int)(blockIdx.x / (unsigned)L.nq);     // index within the queue
+  const int nw = (int)(gridDim.x / (unsigned)L.nq);      // workgroups per queue
 
   int iter = 0;
   while (true) {
-    int u;
+    int i;
     if (L.use_ticket) {
-      if (tid == 0) s_unit = atomicAdd(L.ws, 1);
+      if (tid == 0) s_unit = atomicAdd(L.ws + kWsTicket * x, 1);
       __syncthreads();
-      u = __builtin_amdgcn_readfirstlane(s_unit);
+      i = __builtin_amdgcn_readfirstlane(s_unit);
       __syncthreads();
     } else {
-      u = blockIdx.x + iter * gridDim.x;
+      i = wq + iter * nw;
     }
     ++iter;
-    if (u >= L.total_units) break;
+    int j, q, p;
+    if (!decode_unit(L, x, i, &j, &q, &p)) break;
+    j = __builtin_amdgcn_readfirstlane(j);
+    q = __builtin_amdgcn_readfirstlane(q);
+    p = __builtin_amdgcn_readfirstlane(p);
 
-    const Job& J = job_at(L, find_job(L, u));
-    const int ul = u - J.unit_base;
-    const int ntile = J.tiles_m * J.tiles_n;
-    const int q = ul / ntile;
-    const int p = ul - q * ntile;
+    const Job& J = job_at(L, j);
     const int tmi = p / J.tiles_n;
     const int tni = p - tmi * J.tiles_n;
-    Prod P = prod_at(J, q);
-    P.a0 = __builtin_amdgcn_readfirstlane(P.a0);
-    P.na = __builtin_amdgcn_readfirstlane(P.na);
-    P.b0 = __builtin_amdgcn_readfirstlane(P.b0);
-    P.nb = __builtin_amdgcn_readfirstlane(P.nb);
-    P.c0 = __builtin_amdgcn_readfirstlane(P.c0);
-    P.nc = __builtin_amdgcn_readfirstlane(P.nc);
-    P.fringe = __builtin_amdgcn_readfirstlane(P.fringe);
-
     UnitGeo g;
     g.row0 = tmi * BM;
     g.col0 = tni * BN;
-    g.depth = P.fringe ? J.kr : J.ks;
-    g.interior = (J.flags & 3) == 3 && g.row0 + BM <= J.ms && g.col0 + BN <= J.ns;
-    g.aoff = g.row0 + g.arow + (long long)g.acol * J.lda;
-    g.boff = (long long)(g.col0 + g.bcol) * J.ldb + g.bk;
     g.arow = 2 * (tid & 63);
     g.acol = tid >> 6;
     g.bcol = tid / B_TPC;
     g.bk = B_KPT * (tid % B_TPC);
+    g.interior = (J.flags & 3) == 3 && g.row0 + BM <= J.ms && g.col0 + BN <= J.ns;
+    g.aoff = g.row0 + g.arow + (long long)g.acol * J.lda;
+    g.boff = (long long)(g.col0 + g.bcol) * J.ldb + g.bk;
 
-    f64x4 acc[FM][FN];
-#pragma unroll
-    for (int mi = 0; mi < FM; ++mi)
-#pragma unroll
-      for (int ni = 0; ni < FN; ++ni) acc[mi][ni] = f64x4{0.0, 0.0, 0.0, 0.0};
-
-    contract<CH>(J, P, g, As, Bs, lane, wm, wn, acc);
-
+    // The group's products run back to back on this tile; the workgroup
+    // applies their C updates itself, in order, so only the first update of
+    // the group waits for the previous group and only the last one signals.
     const bool ordered = J.order_base >= 0;
-    int* ctr = L.ws + 2 + (ordered ? J.order_base + p : 0);
-    if (ordered && q > 0) wait_turn(ctr, q, L);
-    scatter_c(J, P, g.row0, g.col0, lane, wm, wn, acc);
+    int* ctr = L.ws + kWsCounters + (ordered ? J.order_base + p : 0);
+    const int q0 = q * J.gsize;
+    const int q1 = min(J.nprod, q0 + J.gsize);
+    for (int qq = q0; qq < q1; ++qq) {
+      Prod P = prod_at(J, qq);
+      P.a0 = __builtin_amdgcn_readfirstlane(P.a0);
+      P.na = __builtin_amdgcn_readfirstlane(P.na);
+      P.b0 = __builtin_amdgcn_readfirstlane(P.b0);
+      P.nb = __builtin_amdgcn_readfirstlane(P.nb);
+      P.c0 = __builtin_amdgcn_readfirstlane(P.c0);
+      P.nc = __builtin_amdgcn_readfirstlane(P.nc);
+      P.fringe = __builtin_amdgcn_readfirstlane(P.fringe);
+      g.depth = __builtin_amdgcn_readfirstlane(P.fringe ? J.kr : J.ks);
+
+      f64x4 acc[FM][FN];
+#pragma unroll
+      for (int mi = 0; mi < FM; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < FN; ++ni) acc[mi][ni] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+      contract<BK, CH>(J, P, g, As, Bs, lane, wm, wn, wave >> 2, acc);
+
+      if (ordered && q > 0 && qq == q0) wait_turn(ctr, q, L);
+      scatter_c(J, P, g.row0, g.col0, lane, wm, wn, acc);
+    }
     if (ordered) pass_turn(ctr, q + 1);
   }
 }

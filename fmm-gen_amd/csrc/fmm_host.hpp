@@ -191,7 +191,7 @@ struct DevInfo {
   long long spin_ticks = 0;
 };
 
-template <int CH>
+template <int BK, int CH>
 inline int dev_info(DevInfo* out) {
   static std::mutex mu;
   static std::vector<DevInfo> cache;
@@ -204,7 +204,7 @@ inline int dev_info(DevInfo* out) {
     int cus = 0, khz = 0, nb = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return kEHip;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, abc_f64_kernel<CH>, NT, 0) != hipSuccess) return kEHip;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, abc_f64_kernel<BK, CH>, NT, 0) != hipSuccess) return kEHip;
     d.cus = cus;
     d.blocks_per_cu = std::max(nb, 1);
     d.spin_ticks = (long long)khz * 1000LL * 5LL;   // 5 s
@@ -249,6 +249,8 @@ inline void add_gemm_job(JobList& L, const Prod* dp, const Term* dt, int m, int 
   j.tiles_m = tiles(m, BM);
   j.tiles_n = tiles(n, BN);
   j.nprod = 1;
+  j.gsize = 1;
+  j.ngroups = 1;
   j.unit_base = L.total_units;
   j.order_base = -1;
   j.flags = vec_flags(A, lda, B, ldb, 1, C, ldc);
@@ -292,6 +294,8 @@ inline void add_fmm_jobs(JobList& L, const Plan& P, const Prod* dp, const Term* 
   j.tiles_m = tiles(ms, BM);
   j.tiles_n = tiles(ns, BN);
   j.nprod = P.R + (kr > 0 ? P.Mx * P.Nx : 0);
+  j.gsize = 1;
+  j.ngroups = j.nprod;
   j.unit_base = L.total_units;
   j.order_base = j.nprod > 1 ? L.counters : -1;
   j.flags = vec_flags(A, lda, B, ldb, (ks & 1) == 0, C, ldc);
@@ -305,17 +309,52 @@ inline void add_fmm_jobs(JobList& L, const Plan& P, const Prod* dp, const Term* 
   L.jobs.push_back(j);
 }
 
-inline size_t ws_ints(const JobList& L) { return L.ordered ? (size_t)(2 + L.counters) : 0; }
+// Group consecutive products of each FMM job into units so that the launch
+// has about kUnitsPerSlot units per resident workgroup: fewer, longer units
+// mean fewer cross-workgroup hand-overs (each costs an agent-scope release and
+// acquire of C), while enough units remain to balance the grid.  A job whose
+// products all fit one unit needs no ordering at all.  Unit bases are
+// recomputed; counters stay reserved (workspace size does not depend on the
+// device).
+#ifndef FMM_UNITS_PER_SLOT
+#define FMM_UNITS_PER_SLOT 0
+#endif
+constexpr int kUnitsPerSlot = FMM_UNITS_PER_SLOT;
+inline void group_products(JobList& L, int slots) {
+  const long long target = kUnitsPerSlot > 0 ? (long long)kUnitsPerSlot * std::max(slots, 1) : (1LL << 40);
+  int base = 0;
+  bool ordered = false;
+  for (Job& j : L.jobs) {
+    const int ntile = j.tiles_m * j.tiles_n;
+    if (j.nprod > 1) {
+      long long ng = (target + ntile - 1) / ntile;
+      ng = std::max(1LL, std::min<long long>(ng, j.nprod));
+      j.gsize = (int)((j.nprod + ng - 1) / ng);
+      j.ngroups = (j.nprod + j.gsize - 1) / j.gsize;
+      if (j.ngroups == 1) j.order_base = -1;
+    }
+    if (j.order_base >= 0) ordered = true;
+    j.unit_base = base;
+    base += j.ngroups * ntile;
+  }
+  L.total_units = base;
+  L.ordered = ordered;
+}
+
+inline size_t ws_ints(const JobList& L) { return L.ordered ? (size_t)(kWsCounters + L.counters) : 0; }
 
 // Launch a job list (<= kInlineJobs jobs, passed by value) on `stream`.
 // `ws` must hold ws_ints(L) ints when L.ordered.
-template <int CH>
-inline int launch_jobs(const JobList& JL, int* ws, hipStream_t stream) {
+template <int BK, int CH>
+inline int launch_jobs_k(JobList JL, int* ws, hipStream_t stream) {
   if (JL.total_units == 0) return kOK;
   if ((int)JL.jobs.size() > kInlineJobs) return kEInval;
   DevInfo di;
-  int rc = dev_info<CH>(&di);
+  int rc = dev_info<BK, CH>(&di);
   if (rc != kOK) return rc;
+  const int slots = di.cus * di.blocks_per_cu;
+  const size_t ws_n = ws_ints(JL);   // counters as reserved by the caller
+  group_products(JL, slots);
   Launch L{};
   L.njobs = (int)JL.jobs.size();
   L.total_units = JL.total_units;
@@ -326,14 +365,29 @@ inline int launch_jobs(const JobList& JL, int* ws, hipStream_t stream) {
   for (size_t i = 0; i < JL.jobs.size(); ++i) L.inl[i] = JL.jobs[i];
   if (JL.ordered) {
     if (!ws) return kEInval;
-    const int n = (int)ws_ints(JL);
+    const int n = (int)ws_n;
     hipLaunchKernelGGL(ws_clear_kernel, dim3(std::min((n + 255) / 256, 1024)), dim3(256), 0, stream, ws, n);
     if (hipGetLastError() != hipSuccess) return kEHip;
   }
-  const int grid = std::max(1, std::min(JL.total_units, di.cus * di.blocks_per_cu));
-  abc_f64_kernel<CH><<<dim3(grid), dim3(NT), 0, stream>>>(L);
+  // One resident workgroup per slot; 8 queues (one per XCD) when the grid
+  // fills every XCD with a whole number of workgroups.
+  int grid = std::max(1, std::min(JL.total_units, slots));
+  L.nq = 1;
+  if (grid >= 8 * kMaxQueues) {
+    grid -= grid % kMaxQueues;
+    L.nq = kMaxQueues;
+  }
+  abc_f64_kernel<BK, CH><<<dim3(grid), dim3(NT), 0, stream>>>(L);
   if (hipGetLastError() != hipSuccess) return kEHip;
   return kOK;
+}
+
+// Kernel variant by the largest operand-sum term count of the launch:
+// BK = 32 with 2-term chunks when every product has at most 2 terms per
+// operand (plain GEMM, Strassen, <2,3,2>), else BK = 16 with 4-term chunks.
+inline int launch_jobs(const JobList& JL, int max_terms, int* ws, hipStream_t stream) {
+  if (max_terms <= 2) return launch_jobs_k<32, 2>(JL, ws, stream);
+  return launch_jobs_k<16, 4>(JL, ws, stream);
 }
 
 }  // namespace fmm

@@ -37,7 +37,7 @@ def test_plan_create_and_info(fmm, rt):
     assert rt.lib().fmm_plan_info(p.handle, *[ct.byref(o) for o in out]) == 0
     assert [o.value for o in out] == [6, 4, 6, 105]
     # 14400^2 x 480: one counter per C tile of the 2400x2400 leaf (19x19) + ticket + flags
-    assert p.workspace_bytes(14400, 14400, 480) == (2 + 19 * 19) * 4
+    assert p.workspace_bytes(14400, 14400, 480) == (264 + 19 * 19) * 4
     # classical-depth problem smaller than the partition: no FMM core, no counters
     assert p.workspace_bytes(40, 40, 40) == 0
     p.close()

@@ -28,7 +28,7 @@ def test_emitted_source(tmp_path, fmm):
     from fmm_gen_amd import codegen
     alg = fmm.build_algorithm(["222"], [1])
     src = codegen.emit_source(alg, "222-1_abc", "abc")
-    assert "#define FMM_CH 2" in src
+    assert "Largest number of A or B terms of one product: 2" in src
     assert "void bl_dgemm_strassen_abc(" in src
     # Strassen's first product in the reference table: (A00 + A11)(B00 + B11)
     assert "M0 = (A(0,0) + A(1,1)) * (B(0,0) + B(1,1)); C(0,0) += M0; C(1,1) += M0" in src
