@@ -43,7 +43,34 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-gemm", action="store_true", help="skip the plain-GEMM comparison")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 1 GiB three times (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain copy of a known size, for scaling rocprofv3 FETCH_SIZE /
+    WRITE_SIZE readings to bytes: each launch reads and writes 1 GiB."""
+    import torch
+    n = (1 << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n * 8, "launches": 3}), flush=True)
+
+
+def load_traffic(workload):
+    """HBM bytes per launch of the FMM kernel for this workload, from the
+    committed rocprofv3 counter summary (profiles/traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+        return t.get(workload, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
 
 
 def sampled_error(A, B, C0, C, nsamp=256, seed=1):
@@ -71,7 +98,7 @@ def sampled_error(A, B, C0, C, nsamp=256, seed=1):
 def cpu_baseline(F, algs, levels):
     """The oracle's C/OpenMP restatement of the reference ABC algorithm (the
     same tables, partition, product order and packing-sum order), timed on
-    this host on a bounded sample: the same algorithm at m=n=k=3072, min of 3."""
+    this host on a bounded sample: the same algorithm at m=n=k=4608, min of 3."""
     import numpy as np
     from oracle import oracle
     alg = F.build_algorithm(algs, levels)
@@ -81,7 +108,7 @@ def cpu_baseline(F, algs, levels):
         ncpu = os.cpu_count() or 1
     threads = int(os.environ.get("BLISLAB_IC_NT", min(ncpu, 16)))
     oracle.set_threads(threads)
-    s = 3072
+    s = 4608
     rng = np.random.default_rng(7)
     A = np.asfortranarray(rng.uniform(-1, 1, (s, s)))
     B = np.asfortranarray(rng.uniform(-1, 1, (s, s)))
@@ -94,11 +121,14 @@ def cpu_baseline(F, algs, levels):
     return {"value": round(2.0 * s ** 3 / best / 1e9, 2), "unit": "GFLOPS", "cores": threads,
             "kind": "port",
             "sample": f"oracle C restatement of the reference ABC {'x'.join(algs)} "
-                      f"L{levels}, m=n=k={s}, FP64, min of 3"}
+                      f"levels {levels}, m=n=k={s}, FP64, seeded U[-1,1), min of 3 runs"}
 
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
               "sharding of this path is described in DESIGN.md and not built.", flush=True)
@@ -167,7 +197,7 @@ def main():
     nnzU, nnzV, nnzW = alg.nnz()
     b_min = 8 * (nnzU * p.ms * p.ks + nnzV * p.ks * p.ns) + 8 * 2 * nnzW * p.ms * p.ns
     roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_F64_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_F64_TFLOPS, 4), "traffic": None,
+            "frac": round(achieved / PEAK_F64_TFLOPS, 4), "traffic": load_traffic(args.workload),
             "kernel": "abc_f64_kernel", "flops_per_launch": f_mfma,
             "algorithmic_bytes_per_launch": b_min, "kernel_ms_mean": round(kern_ms, 4)}
 
