@@ -81,13 +81,15 @@ struct Launch {
   int total_units;
   int use_ticket;           // 1: units are claimed from per-queue tickets in ws
   int nq;                   // unit queues (1 or 8, one per XCD)
+  int local_handover;       // 1: each queue runs on one XCD (see wait_turn)
+  int pad2;
   const Job* jobs;          // device job table, or nullptr -> inl[]
   int* ws;                  // workspace, layout kWs* below
   long long spin_limit;     // bounded wait, in wall_clock64() ticks
   Job inl[kInlineJobs];
 };
 
-enum : int { kErrSpin = 1 };
+enum : int { kErrSpin = 1, kErrPlacement = 2 };
 
 // Workspace layout (ints): queue x's ticket at kWsTicket*x (one 128-byte line
 // each), the error flags at kWsError, the per-tile order counters from
@@ -431,10 +433,34 @@ __device__ __forceinline__ void mfma_slice(const double (*As)[BM], const double 
 #endif
 }
 
+// Hand-over of a C tile between consecutive product units.
+//
+// Reader: an agent-scope acquire (buffer_inv sc1: L1 and non-coherent L2 lines
+// invalidated) -- cheap.  An L1-only invalidate (buffer_inv sc0) was measured
+// not to suffice outside threadgroup-split dispatch: a CU re-reading a tile it
+// had cached returned stale C.
+//
+// Writer: with agent scope the release writes back the whole XCD L2
+// (buffer_wbl2 sc1), which flushes every dirty line of the XCD and measured
+// 8-13% of an FMM launch.  With local_handover all units of a tile run in the
+// queue of one XCD (queue = XCC_ID, verified by a probe at set-up and checked
+// at kernel start), so writer and reader share that L2 and the release only
+// has to wait until the writer's C stores are acknowledged by it
+// (s_waitcnt vmcnt(0)).  The counter is always accessed with agent-scope
+// atomics, which are performed at L2.
+__device__ __forceinline__ void acquire_c() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
+__device__ __forceinline__ void release_c(bool local) {
+  if (local) {
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  }
+}
+
 // Wait (thread 0, bounded) until counter reaches `want`, polling with relaxed
-// loads (an acquiring poll would invalidate this XCD's L2 on every
-// iteration); then every thread acquires once at agent scope so its C reads
-// see the previous product's writes.
+// loads (an acquiring poll would invalidate caches on every iteration); then
+// every thread acquires once so its C reads see the previous product's writes.
 __device__ __forceinline__ void wait_turn(int* ctr, int want, const Launch& L) {
   if (threadIdx.x == 0) {
     if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
@@ -452,13 +478,18 @@ __device__ __forceinline__ void wait_turn(int* ctr, int want, const Launch& L) {
     }
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  acquire_c();
 }
 
-__device__ __forceinline__ void pass_turn(int* ctr, int next) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+__device__ __forceinline__ void pass_turn(int* ctr, int next, const Launch& L) {
+  release_c(L.local_handover != 0);
   __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(ctr, next, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) __hip_atomic_store(ctr, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Hardware XCC (XCD) id of the executing workgroup: HW_REG_XCC_ID, bits 3:0.
+__device__ __forceinline__ int xcc_id() {
+  return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)) & 15;
 }
 
 // C_j = C_j + w_j * acc for every C term of the product, straight from the
@@ -599,6 +630,7 @@ __global__ void __launch_bounds__(NT) abc_f64_kernel(Launch L) {
   const int x = (int)(blockIdx.x % (unsigned)L.nq);
   const int wq = (int)(blockIdx.x / (unsigned)L.nq);     // index within the queue
   const int nw = (int)(gridDim.x / (unsigned)L.nq);      // workgroups per queue
+  if (L.local_handover && tid == 0 && xcc_id() != x) atomicOr(L.ws + kWsError, (int)kErrPlacement);
 
   int iter = 0;
   while (true) {
@@ -661,8 +693,14 @@ __global__ void __launch_bounds__(NT) abc_f64_kernel(Launch L) {
       if (ordered && q > 0 && qq == q0) wait_turn(ctr, q, L);
       scatter_c(J, P, g.row0, g.col0, lane, wm, wn, acc);
     }
-    if (ordered) pass_turn(ctr, q + 1);
+    if (ordered) pass_turn(ctr, q + 1, L);
   }
+}
+
+// Records the XCC id of each workgroup (one launch per device at set-up):
+// the host enables XCD-local hand-over only if workgroup i runs on XCC i % 8.
+__global__ void xcc_probe_kernel(int* out) {
+  if (threadIdx.x == 0) out[blockIdx.x] = xcc_id();
 }
 
 // Zero the ticket, error flag and order counters of a launch (a plain fill

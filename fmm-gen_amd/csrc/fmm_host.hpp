@@ -189,7 +189,26 @@ struct DevInfo {
   int cus = 0;
   int blocks_per_cu = 0;
   long long spin_ticks = 0;
+  bool xcd_round_robin = false;   // workgroup i runs on XCC i % 8 (probed)
 };
+
+// One-time probe: launch 256 workgroups and check that workgroup i reports
+// XCC id i % 8 (the dispatch pattern XCD-local hand-over relies on).
+inline bool probe_xcd_round_robin() {
+  const int n = 256;
+  int* d = nullptr;
+  if (hipMalloc(&d, n * sizeof(int)) != hipSuccess) return false;
+  bool ok = false;
+  xcc_probe_kernel<<<dim3(n), dim3(64), 0, 0>>>(d);
+  std::vector<int> h(n, -1);
+  if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+      hipMemcpy(h.data(), d, n * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess) {
+    ok = true;
+    for (int i = 0; i < n; ++i) ok = ok && h[i] == i % kMaxQueues;
+  }
+  (void)hipFree(d);
+  return ok;
+}
 
 template <int BK, int CH>
 inline int dev_info(DevInfo* out) {
@@ -207,6 +226,7 @@ inline int dev_info(DevInfo* out) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, abc_f64_kernel<BK, CH>, NT, 0) != hipSuccess) return kEHip;
     d.cus = cus;
     d.blocks_per_cu = std::max(nb, 1);
+    d.xcd_round_robin = probe_xcd_round_robin();
     d.spin_ticks = (long long)khz * 1000LL * 5LL;   // 5 s
   }
   *out = d;
@@ -373,9 +393,11 @@ inline int launch_jobs_k(JobList JL, int* ws, hipStream_t stream) {
   // fills every XCD with a whole number of workgroups.
   int grid = std::max(1, std::min(JL.total_units, slots));
   L.nq = 1;
+  L.local_handover = 0;
   if (grid >= 8 * kMaxQueues) {
     grid -= grid % kMaxQueues;
     L.nq = kMaxQueues;
+    L.local_handover = di.xcd_round_robin ? 1 : 0;
   }
   abc_f64_kernel<BK, CH><<<dim3(grid), dim3(NT), 0, stream>>>(L);
   if (hipGetLastError() != hipSuccess) return kEHip;
