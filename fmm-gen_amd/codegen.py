@@ -89,9 +89,15 @@ def max_terms(alg: Algorithm) -> int:
     return best
 
 
+PACK_CONST = {"abc": "FMM_PACK_ABC", "ab": "FMM_PACK_AB", "naive": "FMM_PACK_NAIVE"}
+PACK_REF = {"abc": "meta/abc_gen_mix.py:448", "ab": "meta/ab_gen_mix.py:341",
+            "naive": "meta/naive_gen_mix.py:244"}
+
+
 def emit_source(alg: Algorithm, job: str, pack_type: str) -> str:
-    if pack_type != "abc":
-        raise ValueError(f"pack_type {pack_type!r}: only 'abc' is generated as HIP")
+    if pack_type not in PACK_CONST:
+        raise ValueError(f"pack_type {pack_type!r}: one of {sorted(PACK_CONST)}")
+    pk = PACK_CONST[pack_type]
     Mx, Kx, Nx = alg.dims
     ch = max_terms(alg)
     dims = ", ".join(str(d) for lv in alg.levels for d in lv)
@@ -105,6 +111,7 @@ def emit_source(alg: Algorithm, job: str, pack_type: str) -> str:
 //
 // Largest number of A or B terms of one product: {ch} (the kernel variant is
 // chosen from it: 32-deep slices for at most 2 terms, else 16-deep).
+// Pack type: {pack_type}.
 #include "fmm_capi.hpp"
 
 namespace {{
@@ -131,19 +138,22 @@ extern "C" {{
 
 fmm_plan_t fmm_job_plan(void) {{ return job_plan(); }}
 
-int fmm_job_abc(int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
+int fmm_job_pack(void) {{ return {pk}; }}
+
+int fmm_job_{pack_type}(int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
                 void* ws, size_t ws_bytes, void* stream) {{
   fmm_plan_t p = job_plan();
   if (!p) return FMM_ENOMEM;
-  return fmm_abc(p, FMM_F64, m, n, k, A, lda, B, ldb, C, ldc, ws, ws_bytes, stream);
+  return fmm_run(p, {pk}, FMM_F64, m, n, k, A, lda, B, ldb, C, ldc, ws, ws_bytes, stream);
 }}
 
-// The reference's generated entry (meta/abc_gen_mix.py:448): host pointers,
+// The reference's generated entry ({PACK_REF[pack_type]}): host pointers,
 // C += A*B, no return value.
-void bl_dgemm_strassen_abc(int m, int n, int k, double* XA, int lda, double* XB, int ldb, double* XC, int ldc) {{
+void bl_dgemm_strassen_{pack_type}(int m, int n, int k, double* XA, int lda, double* XB, int ldb, double* XC,
+                                   int ldc) {{
   fmm_plan_t p = job_plan();
-  int rc = p ? fmm_host_abc(p, m, n, k, XA, lda, XB, ldb, XC, ldc) : FMM_ENOMEM;
-  fmm::report("bl_dgemm_strassen_abc", rc);
+  int rc = p ? fmm_host_run(p, {pk}, m, n, k, XA, lda, XB, ldb, XC, ldc) : FMM_ENOMEM;
+  fmm::report("bl_dgemm_strassen_{pack_type}", rc);
 }}
 
 }}  // extern "C"
@@ -211,6 +221,10 @@ int main(int argc, char** argv) {
   CHECK(hipMemcpy(dB, B, nb * sizeof(double), hipMemcpyHostToDevice));
   CHECK(hipMemset(dC, 0, nc * sizeof(double)));
   CHECK(hipMemset(dCref, 0, nc * sizeof(double)));
+  // the job's workspace, taken once outside the timed calls
+  size_t ws_bytes = fmm_workspace_bytes_pack(fmm_job_plan(), fmm_job_pack(), m, n, k);
+  void* ws = NULL;
+  if (ws_bytes) CHECK(hipMalloc(&ws, ws_bytes));
   hipEvent_t e0, e1;
   CHECK(hipEventCreate(&e0));
   CHECK(hipEventCreate(&e1));
@@ -218,10 +232,10 @@ int main(int argc, char** argv) {
   float best_fmm = 1e30f, best_gemm = 1e30f, t;
   for (int r = 0; r < nrepeats; ++r) {
     CHECK(hipEventRecord(e0, 0));
-    int rc = fmm_job_abc(m, n, k, dA, lda, dB, ldb, dC, ldc, NULL, 0, NULL);
+    int rc = fmm_job_@PACK@(m, n, k, dA, lda, dB, ldb, dC, ldc, ws, ws_bytes, NULL);
     CHECK(hipEventRecord(e1, 0));
     CHECK(hipEventSynchronize(e1));
-    if (rc != FMM_OK) { fprintf(stderr, "fmm_job_abc: error %d\n", rc); return 1; }
+    if (rc != FMM_OK) { fprintf(stderr, "fmm_job_@PACK@: error %d\n", rc); return 1; }
     CHECK(hipEventElapsedTime(&t, e0, e1));
     if (t < best_fmm) best_fmm = t;
   }
@@ -258,6 +272,7 @@ int main(int argc, char** argv) {
   CHECK(hipFree(dB));
   CHECK(hipFree(dC));
   CHECK(hipFree(dCref));
+  if (ws) CHECK(hipFree(ws));
   free(A);
   free(B);
   free(C);

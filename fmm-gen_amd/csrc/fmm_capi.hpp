@@ -48,6 +48,39 @@ inline int run_abc(Plan* P, int m, int n, int k, const double* A, int lda, const
   return rc;
 }
 
+// One FMM call of pack type `pack`.  AB / Naive need the workspace of
+// pack_layout; a shape without an FMM core runs the ABC path (plain GEMMs).
+inline int run_fmm(Plan* P, int pack, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                   double* C, int ldc, void* ws, size_t ws_bytes, hipStream_t s) {
+  if (pack != kPackABC && pack != kPackAB && pack != kPackNaive) return kEInval;
+  if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
+  if (m == 0 || n == 0 || k == 0) return kOK;
+  PackLayout Lo;
+  if (pack == kPackABC || !pack_layout(*P, pack, m, n, k, &Lo))
+    return run_abc(P, m, n, k, A, lda, B, ldb, C, ldc, ws, ws_bytes, s);
+  void* w = ws;
+  bool own = false;
+  if (!w) {
+    if (hipMallocAsync(&w, Lo.bytes, s) != hipSuccess) return kENoMem;
+    own = true;
+  } else if (ws_bytes < Lo.bytes) {
+    return kEInval;
+  }
+  int rc = launch_pack(P, pack, Lo, m, n, k, A, lda, B, ldb, C, ldc, w, s);
+  if (own) (void)hipFreeAsync(w, s);
+  return rc;
+}
+
+inline size_t workspace_bytes(Plan* P, int pack, int m, int n, int k) {
+  if (m <= 0 || n <= 0 || k <= 0) return 0;
+  PackLayout Lo;
+  if (pack != kPackABC && pack_layout(*P, pack, m, n, k, &Lo)) return Lo.bytes;
+  JobList JL;
+  add_fmm_jobs(JL, *P, nullptr, nullptr, nullptr, nullptr, m, n, k, nullptr, std::max(m, 1), nullptr,
+               std::max(k, 1), nullptr, std::max(m, 1));
+  return ws_ints(JL) * sizeof(int);
+}
+
 inline int run_gemm(int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
                     hipStream_t s) {
   if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
@@ -161,19 +194,36 @@ int fmm_plan_info(fmm_plan_t plan, int* Mx, int* Kx, int* Nx, int* rank) {
 }
 
 size_t fmm_workspace_bytes(fmm_plan_t plan, int m, int n, int k) {
-  if (!plan || m <= 0 || n <= 0 || k <= 0) return 0;
-  fmm::JobList JL;
-  fmm::add_fmm_jobs(JL, plan->p, nullptr, nullptr, nullptr, nullptr, m, n, k, nullptr, std::max(m, 1), nullptr,
-                    std::max(k, 1), nullptr, std::max(m, 1));
-  return fmm::ws_ints(JL) * sizeof(int);
+  if (!plan) return 0;
+  return fmm::workspace_bytes(&plan->p, fmm::kPackABC, m, n, k);
+}
+
+size_t fmm_workspace_bytes_pack(fmm_plan_t plan, int pack, int m, int n, int k) {
+  if (!plan) return 0;
+  return fmm::workspace_bytes(&plan->p, pack, m, n, k);
+}
+
+int fmm_run(fmm_plan_t plan, int pack, int dtype, int m, int n, int k, const void* A, int lda, const void* B,
+            int ldb, void* C, int ldc, void* ws, size_t ws_bytes, void* stream) {
+  if (!plan) return fmm::kEInval;
+  if (dtype != FMM_F64) return fmm::kEUnsupported;
+  return fmm::run_fmm(&plan->p, pack, m, n, k, static_cast<const double*>(A), lda, static_cast<const double*>(B),
+                      ldb, static_cast<double*>(C), ldc, ws, ws_bytes, static_cast<hipStream_t>(stream));
 }
 
 int fmm_abc(fmm_plan_t plan, int dtype, int m, int n, int k, const void* A, int lda, const void* B, int ldb, void* C,
             int ldc, void* ws, size_t ws_bytes, void* stream) {
-  if (!plan) return fmm::kEInval;
-  if (dtype != FMM_F64) return fmm::kEUnsupported;
-  return fmm::run_abc(&plan->p, m, n, k, static_cast<const double*>(A), lda, static_cast<const double*>(B), ldb,
-                      static_cast<double*>(C), ldc, ws, ws_bytes, static_cast<hipStream_t>(stream));
+  return fmm_run(plan, FMM_PACK_ABC, dtype, m, n, k, A, lda, B, ldb, C, ldc, ws, ws_bytes, stream);
+}
+
+int fmm_ab(fmm_plan_t plan, int dtype, int m, int n, int k, const void* A, int lda, const void* B, int ldb, void* C,
+           int ldc, void* ws, size_t ws_bytes, void* stream) {
+  return fmm_run(plan, FMM_PACK_AB, dtype, m, n, k, A, lda, B, ldb, C, ldc, ws, ws_bytes, stream);
+}
+
+int fmm_naive(fmm_plan_t plan, int dtype, int m, int n, int k, const void* A, int lda, const void* B, int ldb,
+              void* C, int ldc, void* ws, size_t ws_bytes, void* stream) {
+  return fmm_run(plan, FMM_PACK_NAIVE, dtype, m, n, k, A, lda, B, ldb, C, ldc, ws, ws_bytes, stream);
 }
 
 int fmm_gemm(int dtype, int m, int n, int k, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
@@ -193,15 +243,20 @@ int fmm_ws_status(const void* ws, int* flags) {
   return v ? fmm::kESpin : fmm::kOK;
 }
 
-int fmm_host_abc(fmm_plan_t plan, int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C,
-                 int ldc) {
+int fmm_host_run(fmm_plan_t plan, int pack, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                 double* C, int ldc) {
   if (!plan) return fmm::kEInval;
   return fmm::host_call(m, n, k, A, lda, B, ldb, C, ldc,
                         [&](const double* a, int la, const double* b, int lb, double* c, int lc) {
-                          int rc = fmm::run_abc(&plan->p, m, n, k, a, la, b, lb, c, lc, nullptr, 0, nullptr);
+                          int rc = fmm::run_fmm(&plan->p, pack, m, n, k, a, la, b, lb, c, lc, nullptr, 0, nullptr);
                           if (rc == fmm::kOK && hipDeviceSynchronize() != hipSuccess) rc = fmm::kEHip;
                           return rc;
                         });
+}
+
+int fmm_host_abc(fmm_plan_t plan, int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C,
+                 int ldc) {
+  return fmm_host_run(plan, FMM_PACK_ABC, m, n, k, A, lda, B, ldb, C, ldc);
 }
 
 void bl_dgemm(int m, int n, int k, double* XA, int lda, double* XB, int ldb, double* XC, int ldc) {

@@ -69,12 +69,14 @@ struct Job {
   int ngroups;              // ceil(nprod / gsize)
   int unit_base;            // first unit of this job in the launch
   int order_base;           // first order counter, <0 when ngroups == 1
-  int flags;                // bit0/1/2: A/B/C allow 16-byte vector access
+  int flags;                // bit0/1/2: A/B/C allow 16-byte vector access;
+                            // kJobStore: C = w*AB instead of C += w*AB
   const Prod* prods;
   const Term* terms;
 };
 
 constexpr int kInlineJobs = 4;
+constexpr int kJobStore = 8;
 
 struct Launch {
   int njobs;
@@ -492,12 +494,14 @@ __device__ __forceinline__ int xcc_id() {
   return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)) & 15;
 }
 
-// C_j = C_j + w_j * acc for every C term of the product, straight from the
+// C_j = C_j + w_j * acc for every C term of the product (C_j = w_j * acc for
+// a kJobStore job, whose C is a product workspace), straight from the
 // accumulators: for each element v a wave touches 4 columns x 16 consecutive
 // rows, i.e. four full 128-byte runs of the column-major C.
 __device__ __forceinline__ void scatter_c(const Job& J, const Prod& P, int row0, int col0, int lane, int wm,
                                           int wn, const f64x4 (&acc)[FM][FN]) {
   const int l16 = lane & 15, lk = lane >> 4;
+  const bool store = (J.flags & kJobStore) != 0;
   for (int t = 0; t < P.nc; ++t) {
     const Term tm = term_at(J, P.c0 + t);
     const double w = tm.coef;
@@ -514,7 +518,7 @@ __device__ __forceinline__ void scatter_c(const Job& J, const Prod& P, int row0,
           const int col = col0 + wn * WTN + ni * 16 + lk + 4 * e;
           if (col < J.ns) {
             gdouble* c = cb + row + (long long)col * J.ldc;
-            *c = __builtin_fma(w, v[e], *c);
+            *c = store ? w * v[e] : __builtin_fma(w, v[e], *c);
           }
         }
       }
@@ -694,6 +698,153 @@ __global__ void __launch_bounds__(NT) abc_f64_kernel(Launch L) {
       scatter_c(J, P, g.row0, g.col0, lane, wm, wn, acc);
     }
     if (ordered) pass_turn(ctr, q + 1, L);
+  }
+}
+
+// --- element-wise kernels of the AB and Naive pack types --------------------
+//
+// Both walk one leaf block (rows x cols, column-major) per grid.y index in
+// tiles of kEwRows rows x kEwCols columns: a wave covers 128 consecutive rows
+// of one column (two per lane), the 4 waves of a workgroup take every 4th
+// column of the tile.
+constexpr int kEwThreads = 256;
+constexpr int kEwRows = 128;
+constexpr int kEwCols = 16;
+
+// Operand sums of the Naive pack type (meta/naive_gen_mix.py:60-137): for
+// product r, Y_r = sum_t u_t X(bi_t, bj_t) with the arithmetic of the ABC
+// kernel's packing (s = u0*x0, then s = fma(u_t, x_t, s)), so the sums equal
+// the ones the ABC kernel forms in LDS bit for bit.  Block (bi, bj) of X sits
+// at X + bi*rows + bj*cols*ldx; Y_r is block (r % ny, r / ny) of Y.
+struct SumJob {
+  const double* X;
+  double* Y;
+  long long ldx, ldy;
+  int rows, cols;      // leaf block extent
+  int ny;              // blocks per block column of Y
+  int side;            // 0: the A terms of each product, 1: the B terms
+  int vec;             // 1: 16-byte accesses allowed (even rows/ld, aligned)
+  int pad;
+  const Prod* prods;
+  const Term* terms;
+};
+
+__device__ __forceinline__ void ew_tile(int rows, int cols, int* r0, int* c0) {
+  const int tr = (rows + kEwRows - 1) / kEwRows;
+  const int t = (int)blockIdx.x;
+  *r0 = (t % tr) * kEwRows;
+  *c0 = (t / tr) * kEwCols;
+  (void)cols;
+}
+
+__global__ void __launch_bounds__(kEwThreads) presum_kernel(SumJob S) {
+  const int r = (int)blockIdx.y;
+  int r0, c0;
+  ew_tile(S.rows, S.cols, &r0, &c0);
+  if (c0 >= S.cols) return;
+  const Prod* pp = S.prods + r;
+  const int t0 = S.side ? pp->b0 : pp->a0;
+  const int nt = S.side ? pp->nb : pp->na;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = r0 + 2 * lane;
+  gdouble* y = gptr(S.Y) + (long long)(r % S.ny) * S.rows + (long long)(r / S.ny) * S.cols * S.ldy;
+  for (int c = c0 + wave; c < min(c0 + kEwCols, S.cols); c += kEwThreads / 64) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int t = 0; t < nt; ++t) {
+      const Term tm = S.terms[t0 + t];
+      gcdouble* x = (gcdouble*)S.X + (long long)tm.bi * S.rows + (long long)tm.bj * S.cols * S.ldx +
+                    (long long)c * S.ldx;
+      double x0 = 0.0, x1 = 0.0;
+      if (S.vec && i + 1 < S.rows) {
+        const f64x2 v = *(gcf64x2*)(x + i);
+        x0 = v.x;
+        x1 = v.y;
+      } else {
+        if (i < S.rows) x0 = x[i];
+        if (i + 1 < S.rows) x1 = x[i + 1];
+      }
+      s0 = t == 0 ? tm.coef * x0 : __builtin_fma(tm.coef, x0, s0);
+      s1 = t == 0 ? tm.coef * x1 : __builtin_fma(tm.coef, x1, s1);
+    }
+    gdouble* yc = y + (long long)c * S.ldy;
+    if (S.vec && i + 1 < S.rows) {
+      f64x2 v;
+      v.x = s0;
+      v.y = s1;
+      *(__attribute__((address_space(1))) f64x2*)(yc + i) = v;
+    } else {
+      if (i < S.rows) yc[i] = s0;
+      if (i + 1 < S.rows) yc[i + 1] = s1;
+    }
+  }
+}
+
+// M_Add of the AB and Naive pack types (meta/naive_gen_mix.py:244-361,
+// meta/ab_gen_mix.py:341-372): for C block b = I*Nx + J (grid.y),
+// C_b = fma(w, M_r, C_b) over the list cptr[b]..cptr[b+1] in product order --
+// the C update sequence of the ABC kernel, so the results agree bit for bit.
+// M_r is block (r % mx, r / mx) of M; list entries with r >= nlive are skipped.
+struct AddJob {
+  const double* M;
+  double* C;
+  long long ldm, ldc;
+  int rows, cols;      // leaf block extent (ms x ns)
+  int mx, nx;          // C blocks per block column / row
+  int nlive;           // products present in M
+  int vec;
+  const int* cptr;
+  const int* cprod;
+  const double* cw;
+};
+
+__global__ void __launch_bounds__(kEwThreads) madd_kernel(AddJob A) {
+  const int b = (int)blockIdx.y;
+  int r0, c0;
+  ew_tile(A.rows, A.cols, &r0, &c0);
+  if (c0 >= A.cols) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = r0 + 2 * lane;
+  const int l0 = A.cptr[b], l1 = A.cptr[b + 1];
+  gdouble* cb = gptr(A.C) + (long long)(b / A.nx) * A.rows + (long long)(b % A.nx) * A.cols * A.ldc;
+  for (int c = c0 + wave; c < min(c0 + kEwCols, A.cols); c += kEwThreads / 64) {
+    gdouble* cc = cb + (long long)c * A.ldc;
+    const bool v2 = A.vec && i + 1 < A.rows;
+    double c0v = 0.0, c1v = 0.0;
+    if (v2) {
+      const f64x2 v = *(gcf64x2*)(cc + i);
+      c0v = v.x;
+      c1v = v.y;
+    } else {
+      if (i < A.rows) c0v = cc[i];
+      if (i + 1 < A.rows) c1v = cc[i + 1];
+    }
+    for (int l = l0; l < l1; ++l) {
+      const int r = A.cprod[l];
+      if (r >= A.nlive) continue;
+      const double w = A.cw[l];
+      gcdouble* m = (gcdouble*)A.M + (long long)(r % A.mx) * A.rows + (long long)(r / A.mx) * A.cols * A.ldm +
+                    (long long)c * A.ldm;
+      double m0 = 0.0, m1 = 0.0;
+      if (v2) {
+        const f64x2 v = *(gcf64x2*)(m + i);
+        m0 = v.x;
+        m1 = v.y;
+      } else {
+        if (i < A.rows) m0 = m[i];
+        if (i + 1 < A.rows) m1 = m[i + 1];
+      }
+      c0v = __builtin_fma(w, m0, c0v);
+      c1v = __builtin_fma(w, m1, c1v);
+    }
+    if (v2) {
+      f64x2 v;
+      v.x = c0v;
+      v.y = c1v;
+      *(__attribute__((address_space(1))) f64x2*)(cc + i) = v;
+    } else {
+      if (i < A.rows) cc[i] = c0v;
+      if (i + 1 < A.rows) cc[i + 1] = c1v;
+    }
   }
 }
 

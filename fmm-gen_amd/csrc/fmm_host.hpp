@@ -32,17 +32,61 @@ enum Status : int {
 constexpr int kMR = 8;
 constexpr int kNR = 4;
 
+// Host vector with lazily created copies on each device ordinal.
+template <class T>
+struct DevVec {
+  std::vector<T> h;
+  std::vector<T*> d;
+  int get(int dev, T** out) {
+    if ((int)d.size() <= dev) d.resize(dev + 1, nullptr);
+    if (!d[dev]) {
+      T* p = nullptr;
+      const size_t bytes = sizeof(T) * std::max<size_t>(h.size(), 1);
+      if (hipMalloc(&p, bytes) != hipSuccess) return kENoMem;
+      if (!h.empty() && hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(p);
+        return kEHip;
+      }
+      d[dev] = p;
+    }
+    *out = d[dev];
+    return kOK;
+  }
+  void release() {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (size_t i = 0; i < d.size(); ++i)
+      if (d[i]) {
+        (void)hipSetDevice((int)i);
+        (void)hipFree(d[i]);
+      }
+    (void)hipSetDevice(cur);
+    d.clear();
+  }
+};
+
+// A product table: products with their term ranges.
+struct Table {
+  DevVec<Prod> prods;
+  DevVec<Term> terms;
+};
+
 struct Plan {
   int nlev = 0;
   std::vector<int> dims;   // 3*nlev
   int Mx = 1, Kx = 1, Nx = 1, R = 0;
   int max_nnz = 0;         // largest per-product source count (A or B)
-  std::vector<Prod> prods; // R products, then Mx*Nx k-fringe products
-  std::vector<Term> terms;
-  // Device copies, one per device ordinal (created on first use there).
+  // ABC: R products, then Mx*Nx k-fringe products.
+  Table main;
+  // Naive: product r reads workspace blocks S_r, T_r and writes M_r.
+  Table diag;
+  // AB: product r reads its A/B terms as in ABC and writes M_r.
+  Table ab;
+  // For every C block (I*Nx + J, grid order) the products feeding it, in
+  // product order, with their W coefficients (the M_Add lists).
+  DevVec<int> cptr, cprod;
+  DevVec<double> cw;
   std::mutex mu;
-  std::vector<Prod*> d_prods;
-  std::vector<Term*> d_terms;
 };
 
 // Decode composed row `row` of table `which` (0: U -> A(i,p), 1: V -> B(p,j),
@@ -82,94 +126,154 @@ inline int build_plan(Plan* P, int nlev, const int* dims, int R, const double* U
       (long long)P->Mx * P->Nx > (1 << 20))
     return kEInval;
   P->R = R;
-  P->prods.clear();
-  P->terms.clear();
+  std::vector<Prod>& prods = P->main.prods.h;
+  std::vector<Term>& terms = P->main.terms.h;
+  prods.clear();
+  terms.clear();
   P->max_nnz = 0;
   const int rowsU = P->Mx * P->Kx, rowsV = P->Kx * P->Nx, rowsW = P->Mx * P->Nx;
   for (int r = 0; r < R; ++r) {
     Prod pr{};
     auto collect = [&](const double* T, int rows, int which, int* start, int* cnt) {
-      *start = (int)P->terms.size();
+      *start = (int)terms.size();
       for (int a = 0; a < rows; ++a) {
         const double v = T[(size_t)a * R + r];
         if (v == 0.0) continue;
         Term t;
         t.coef = v;
         decode_row(P->dims, which, a, &t.bi, &t.bj);
-        P->terms.push_back(t);
+        terms.push_back(t);
       }
-      *cnt = (int)P->terms.size() - *start;
+      *cnt = (int)terms.size() - *start;
     };
     collect(U, rowsU, 0, &pr.a0, &pr.na);
     collect(V, rowsV, 1, &pr.b0, &pr.nb);
     collect(W, rowsW, 2, &pr.c0, &pr.nc);
     pr.fringe = 0;
     P->max_nnz = std::max(P->max_nnz, std::max(pr.na, pr.nb));
-    P->prods.push_back(pr);
+    prods.push_back(pr);
   }
   // k-fringe: C(I,J) += A(I, Kx) B(Kx, J) over depth kr, as one-term products
   // ordered after all R products (bl_dgemm_util.c:325-333 "kr > 0" branch).
   for (int I = 0; I < P->Mx; ++I)
     for (int J = 0; J < P->Nx; ++J) {
       Prod pr{};
-      pr.a0 = (int)P->terms.size();
-      P->terms.push_back(Term{1.0, I, P->Kx});
+      pr.a0 = (int)terms.size();
+      terms.push_back(Term{1.0, I, P->Kx});
       pr.na = 1;
-      pr.b0 = (int)P->terms.size();
-      P->terms.push_back(Term{1.0, P->Kx, J});
+      pr.b0 = (int)terms.size();
+      terms.push_back(Term{1.0, P->Kx, J});
       pr.nb = 1;
-      pr.c0 = (int)P->terms.size();
-      P->terms.push_back(Term{1.0, I, J});
+      pr.c0 = (int)terms.size();
+      terms.push_back(Term{1.0, I, J});
       pr.nc = 1;
       pr.fringe = 1;
-      P->prods.push_back(pr);
+      prods.push_back(pr);
     }
+  // Naive and AB tables.  The workspaces hold one leaf block per product,
+  // side by side: S_r, T_r and M_r are block (0, r) of S (ld >= ms), T
+  // (ld >= ks) and M (ld >= ms).  The Mx*Nx k-fringe products follow as
+  // products R + I*Nx + J, reading A and B and writing M block R + I*Nx + J.
+  P->diag.prods.h.clear();
+  P->diag.terms.h.clear();
+  P->ab.prods.h.clear();
+  P->ab.terms.h.clear();
+  for (int r = 0; r < R; ++r) {
+    std::vector<Term>& dt = P->diag.terms.h;
+    Prod d{};
+    d.a0 = (int)dt.size();
+    dt.push_back(Term{1.0, 0, r});
+    d.na = 1;
+    d.b0 = (int)dt.size();
+    dt.push_back(Term{1.0, 0, r});
+    d.nb = 1;
+    d.c0 = (int)dt.size();
+    dt.push_back(Term{1.0, 0, r});
+    d.nc = 1;
+    P->diag.prods.h.push_back(d);
+
+    std::vector<Term>& at = P->ab.terms.h;
+    const Prod& o = prods[r];
+    Prod a{};
+    a.a0 = (int)at.size();
+    for (int t = 0; t < o.na; ++t) at.push_back(terms[o.a0 + t]);
+    a.na = o.na;
+    a.b0 = (int)at.size();
+    for (int t = 0; t < o.nb; ++t) at.push_back(terms[o.b0 + t]);
+    a.nb = o.nb;
+    a.c0 = (int)at.size();
+    at.push_back(Term{1.0, 0, r});
+    a.nc = 1;
+    P->ab.prods.h.push_back(a);
+  }
+  for (Table* T : {&P->diag, &P->ab})
+    for (int cb = 0; cb < P->Mx * P->Nx; ++cb) {
+      std::vector<Term>& t = T->terms.h;
+      Prod pr{};
+      pr.a0 = (int)t.size();
+      t.push_back(Term{1.0, cb / P->Nx, P->Kx});
+      pr.na = 1;
+      pr.b0 = (int)t.size();
+      t.push_back(Term{1.0, P->Kx, cb % P->Nx});
+      pr.nb = 1;
+      pr.c0 = (int)t.size();
+      t.push_back(Term{1.0, 0, R + cb});
+      pr.nc = 1;
+      pr.fringe = 1;
+      T->prods.h.push_back(pr);
+    }
+  // M_Add lists per C block.
+  P->cptr.h.assign(1, 0);
+  P->cprod.h.clear();
+  P->cw.h.clear();
+  for (int cb = 0; cb < P->Mx * P->Nx; ++cb) {
+    const int I = cb / P->Nx, J = cb % P->Nx;
+    for (int r = 0; r < R; ++r) {
+      const Prod& o = prods[r];
+      for (int t = 0; t < o.nc; ++t) {
+        const Term& c = terms[o.c0 + t];
+        if (c.bi == I && c.bj == J) {
+          P->cprod.h.push_back(r);
+          P->cw.h.push_back(c.coef);
+        }
+      }
+    }
+    P->cprod.h.push_back(R + cb);   // its k-fringe product, last (as in ABC)
+    P->cw.h.push_back(1.0);
+    P->cptr.h.push_back((int)P->cprod.h.size());
+  }
   return kOK;
 }
 
-inline int device_tables(Plan* P, Prod** dp, Term** dt) {
+inline int device_tables(Plan* P, Table& T, Prod** dp, Term** dt) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return kEHip;
   std::lock_guard<std::mutex> g(P->mu);
-  if ((int)P->d_prods.size() <= dev) {
-    P->d_prods.resize(dev + 1, nullptr);
-    P->d_terms.resize(dev + 1, nullptr);
-  }
-  if (!P->d_prods[dev]) {
-    Prod* a = nullptr;
-    Term* b = nullptr;
-    if (hipMalloc(&a, sizeof(Prod) * P->prods.size()) != hipSuccess) return kENoMem;
-    if (hipMalloc(&b, sizeof(Term) * P->terms.size()) != hipSuccess) {
-      (void)hipFree(a);
-      return kENoMem;
-    }
-    if (hipMemcpy(a, P->prods.data(), sizeof(Prod) * P->prods.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b, P->terms.data(), sizeof(Term) * P->terms.size(), hipMemcpyHostToDevice) != hipSuccess) {
-      (void)hipFree(a);
-      (void)hipFree(b);
-      return kEHip;
-    }
-    P->d_prods[dev] = a;
-    P->d_terms[dev] = b;
-  }
-  *dp = P->d_prods[dev];
-  *dt = P->d_terms[dev];
-  return kOK;
+  int rc = T.prods.get(dev, dp);
+  if (rc == kOK) rc = T.terms.get(dev, dt);
+  return rc;
+}
+
+inline int device_tables(Plan* P, Prod** dp, Term** dt) { return device_tables(P, P->main, dp, dt); }
+
+inline int device_clists(Plan* P, int** ptr, int** prod, double** w) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return kEHip;
+  std::lock_guard<std::mutex> g(P->mu);
+  int rc = P->cptr.get(dev, ptr);
+  if (rc == kOK) rc = P->cprod.get(dev, prod);
+  if (rc == kOK) rc = P->cw.get(dev, w);
+  return rc;
 }
 
 inline void release_plan(Plan* P) {
-  for (size_t d = 0; d < P->d_prods.size(); ++d) {
-    if (P->d_prods[d] || P->d_terms[d]) {
-      int cur = 0;
-      (void)hipGetDevice(&cur);
-      (void)hipSetDevice((int)d);
-      if (P->d_prods[d]) (void)hipFree(P->d_prods[d]);
-      if (P->d_terms[d]) (void)hipFree(P->d_terms[d]);
-      (void)hipSetDevice(cur);
-    }
+  for (Table* T : {&P->main, &P->diag, &P->ab}) {
+    T->prods.release();
+    T->terms.release();
   }
-  P->d_prods.clear();
-  P->d_terms.clear();
+  P->cptr.release();
+  P->cprod.release();
+  P->cw.release();
 }
 
 // The plain-GEMM plan: one product, A(0,0) B(0,0) -> C(0,0).
@@ -397,7 +501,7 @@ inline int launch_jobs_k(JobList JL, int* ws, hipStream_t stream) {
   if (grid >= 8 * kMaxQueues) {
     grid -= grid % kMaxQueues;
     L.nq = kMaxQueues;
-    L.local_handover = di.xcd_round_robin ? 1 : 0;
+    L.local_handover = JL.ordered && di.xcd_round_robin ? 1 : 0;
   }
   abc_f64_kernel<BK, CH><<<dim3(grid), dim3(NT), 0, stream>>>(L);
   if (hipGetLastError() != hipSuccess) return kEHip;
@@ -410,6 +514,170 @@ inline int launch_jobs_k(JobList JL, int* ws, hipStream_t stream) {
 inline int launch_jobs(const JobList& JL, int max_terms, int* ws, hipStream_t stream) {
   if (max_terms <= 2) return launch_jobs_k<32, 2>(JL, ws, stream);
   return launch_jobs_k<16, 4>(JL, ws, stream);
+}
+
+// --- AB and Naive pack types ---------------------------------------------
+//
+// Pack types of the reference (meta/ab_gen_mix.py, meta/naive_gen_mix.py):
+//   ABC    operand sums formed while packing, every product updates its C
+//          blocks directly (ordered per tile);
+//   AB     operand sums formed while packing, product r stored to M_r, then
+//          C_j += w_j M_r (M_Add);
+//   Naive  S_r and T_r summed into workspaces first, M_r = S_r T_r by the
+//          plain GEMM path, then M_Add.
+// AB and Naive need no ordering inside the product launch (every product owns
+// its M_r); the price is the M (and S, T) traffic through HBM.  All three
+// apply the same C updates in the same order with the same arithmetic, so
+// they agree bit for bit.
+enum Pack : int { kPackABC = 0, kPackAB = 1, kPackNaive = 2 };
+
+struct PackLayout {
+  int md = 0, kd = 0, nd = 0, ms = 0, ks = 0, ns = 0, kr = 0;
+  int nlive = 0;                       // M blocks written (R [+ Mx*Nx])
+  long long ldm = 0, lds = 0, ldt = 0; // leading dimensions of M, S, T
+  size_t m_off = 0, s_off = 0, t_off = 0, bytes = 0;
+};
+
+inline size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Workspace layout of an AB / Naive call; false when the shape has no FMM
+// core (then the call is the ABC path's plain GEMMs).
+inline bool pack_layout(const Plan& P, int pack, int m, int n, int k, PackLayout* L) {
+  PackLayout o;
+  o.md = m - m % (P.Mx * kMR);
+  o.kd = k - k % P.Kx;
+  o.nd = n - n % (P.Nx * kNR);
+  o.ms = o.md / P.Mx;
+  o.ks = o.kd / P.Kx;
+  o.ns = o.nd / P.Nx;
+  o.kr = k - o.kd;
+  if (o.ms <= 0 || o.ns <= 0 || o.ks <= 0) return false;
+  o.nlive = P.R + (o.kr > 0 ? P.Mx * P.Nx : 0);
+  o.ldm = o.ms;                        // ms % 8 == 0: 16-byte columns
+  o.lds = o.ms;
+  o.ldt = o.ks + (o.ks & 1);           // even, so T columns stay 16-byte aligned
+  size_t off = 0;
+  o.m_off = off;
+  off += round256(sizeof(double) * (size_t)o.ldm * o.ns * o.nlive);
+  if (pack == kPackNaive) {
+    o.s_off = off;
+    off += round256(sizeof(double) * (size_t)o.lds * o.ks * P.R);
+    o.t_off = off;
+    off += round256(sizeof(double) * (size_t)o.ldt * o.ns * P.R);
+  }
+  o.bytes = off;
+  *L = o;
+  return true;
+}
+
+inline int ew_grid(int rows, int cols) { return tiles(rows, kEwRows) * tiles(cols, kEwCols); }
+
+// One AB or Naive call C += A*B on `stream`: [presum A, presum B,] one product
+// launch (the products into M, the k-fringe products into M, the right and
+// bottom fringe GEMMs straight into C), then M_Add into the C core.
+inline int launch_pack(Plan* P, int pack, const PackLayout& Lo, int m, int n, int k, const double* A,
+                       long long lda, const double* B, long long ldb, double* C, long long ldc, void* ws,
+                       hipStream_t stream) {
+  Prod *dp, *xp, *ip;
+  Term *dt, *xt, *it;
+  int rc = device_tables(P, P->main, &dp, &dt);
+  if (rc == kOK) rc = device_tables(P, pack == kPackNaive ? P->diag : P->ab, &xp, &xt);
+  if (rc == kOK) rc = device_tables(identity_plan(), &ip, &it);
+  int *cptr, *cprod;
+  double* cw;
+  if (rc == kOK) rc = device_clists(P, &cptr, &cprod, &cw);
+  if (rc != kOK) return rc;
+  char* w = static_cast<char*>(ws);
+  double* M = reinterpret_cast<double*>(w + Lo.m_off);
+  const double* pa = A;
+  const double* pb = B;
+  long long la = lda, lb = ldb;
+  int b_even = (Lo.ks & 1) == 0;
+  if (pack == kPackNaive) {
+    double* S = reinterpret_cast<double*>(w + Lo.s_off);
+    double* T = reinterpret_cast<double*>(w + Lo.t_off);
+    SumJob sa{};
+    sa.X = A;
+    sa.Y = S;
+    sa.ldx = lda;
+    sa.ldy = Lo.lds;
+    sa.rows = Lo.ms;
+    sa.cols = Lo.ks;
+    sa.ny = 1;
+    sa.side = 0;
+    sa.vec = ((uintptr_t)A & 15) == 0 && (lda & 1) == 0 && ((uintptr_t)S & 15) == 0;
+    sa.prods = dp;
+    sa.terms = dt;
+    presum_kernel<<<dim3(ew_grid(Lo.ms, Lo.ks), P->R), dim3(kEwThreads), 0, stream>>>(sa);
+    if (hipGetLastError() != hipSuccess) return kEHip;
+    SumJob sb = sa;
+    sb.X = B;
+    sb.Y = T;
+    sb.ldx = ldb;
+    sb.ldy = Lo.ldt;
+    sb.rows = Lo.ks;
+    sb.cols = Lo.ns;
+    sb.side = 1;
+    sb.vec = ((uintptr_t)B & 15) == 0 && (ldb & 1) == 0 && b_even && ((uintptr_t)T & 15) == 0;
+    presum_kernel<<<dim3(ew_grid(Lo.ks, Lo.ns), P->R), dim3(kEwThreads), 0, stream>>>(sb);
+    if (hipGetLastError() != hipSuccess) return kEHip;
+    pa = S;
+    la = Lo.lds;
+    pb = T;
+    lb = Lo.ldt;
+    b_even = 1;   // T_r starts at column r*ns of an even-ld matrix
+  }
+  JobList JL;
+  add_gemm_job(JL, ip, it, m, n - Lo.nd, k, A, lda, B + (long long)Lo.nd * ldb, ldb, C + (long long)Lo.nd * ldc,
+               ldc);
+  add_gemm_job(JL, ip, it, m - Lo.md, Lo.nd, k, A + Lo.md, lda, B, ldb, C + Lo.md, ldc);
+  auto add_m_job = [&](const double* a, long long ja, const double* b, long long jb, int even, int kr, int nprod,
+                       const Prod* prods, const Term* terms) {
+    Job j{};
+    j.A = a;
+    j.B = b;
+    j.C = M;
+    j.lda = ja;
+    j.ldb = jb;
+    j.ldc = Lo.ldm;
+    j.ms = Lo.ms;
+    j.ns = Lo.ns;
+    j.ks = Lo.ks;
+    j.kr = kr;
+    j.tiles_m = tiles(Lo.ms, BM);
+    j.tiles_n = tiles(Lo.ns, BN);
+    j.nprod = nprod;
+    j.gsize = 1;
+    j.ngroups = nprod;
+    j.unit_base = JL.total_units;
+    j.order_base = -1;
+    j.flags = vec_flags(a, ja, b, jb, even, M, Lo.ldm) | kJobStore;
+    j.prods = prods;
+    j.terms = terms;
+    JL.total_units += nprod * j.tiles_m * j.tiles_n;
+    JL.jobs.push_back(j);
+  };
+  add_m_job(pa, la, pb, lb, b_even, 0, P->R, xp, xt);
+  if (Lo.kr > 0) add_m_job(A, lda, B, ldb, (Lo.ks & 1) == 0, Lo.kr, P->Mx * P->Nx, xp + P->R, xt);
+  rc = launch_jobs(JL, pack == kPackNaive ? 1 : P->max_nnz, nullptr, stream);
+  if (rc != kOK) return rc;
+  AddJob ad{};
+  ad.M = M;
+  ad.C = C;
+  ad.ldm = Lo.ldm;
+  ad.ldc = ldc;
+  ad.rows = Lo.ms;
+  ad.cols = Lo.ns;
+  ad.mx = 1;
+  ad.nx = P->Nx;
+  ad.nlive = Lo.nlive;
+  ad.vec = ((uintptr_t)C & 15) == 0 && (ldc & 1) == 0 && ((uintptr_t)M & 15) == 0;
+  ad.cptr = cptr;
+  ad.cprod = cprod;
+  ad.cw = cw;
+  madd_kernel<<<dim3(ew_grid(Lo.ms, Lo.ns), P->Mx * P->Nx), dim3(kEwThreads), 0, stream>>>(ad);
+  if (hipGetLastError() != hipSuccess) return kEHip;
+  return kOK;
 }
 
 }  // namespace fmm

@@ -23,12 +23,14 @@ LIB_DIR = os.path.join(PKG_DIR, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libfmm_amd.so")
 
 FMM_F64 = 0
+PACKS = {"abc": 0, "ab": 1, "naive": 2}
 STATUS = {0: "FMM_OK", -1: "FMM_EINVAL", -2: "FMM_EHIP", -3: "FMM_ENOMEM",
           -4: "FMM_EUNSUPPORTED", -5: "FMM_ESPIN"}
 
 EXPORTS = ("fmm_plan_create", "fmm_plan_destroy", "fmm_plan_info", "fmm_workspace_bytes",
-           "fmm_abc", "fmm_gemm", "fmm_ws_status", "fmm_host_abc", "bl_dgemm",
-           "bl_dgemm_ref", "bl_finalize")
+           "fmm_workspace_bytes_pack", "fmm_run", "fmm_abc", "fmm_ab", "fmm_naive", "fmm_gemm",
+           "fmm_ws_status", "fmm_host_abc", "fmm_host_run", "bl_dgemm", "bl_dgemm_ref",
+           "bl_finalize")
 
 
 class FmmError(RuntimeError):
@@ -48,8 +50,15 @@ def _declare(L, has_job: bool = False):
     L.fmm_plan_info.restype = i
     L.fmm_workspace_bytes.argtypes = [vp, i, i, i]
     L.fmm_workspace_bytes.restype = ct.c_size_t
-    L.fmm_abc.argtypes = [vp, i, i, i, i, vp, i, vp, i, vp, i, vp, ct.c_size_t, vp]
-    L.fmm_abc.restype = i
+    L.fmm_workspace_bytes_pack.argtypes = [vp, i, i, i, i]
+    L.fmm_workspace_bytes_pack.restype = ct.c_size_t
+    for name in ("fmm_abc", "fmm_ab", "fmm_naive"):
+        getattr(L, name).argtypes = [vp, i, i, i, i, vp, i, vp, i, vp, i, vp, ct.c_size_t, vp]
+        getattr(L, name).restype = i
+    L.fmm_run.argtypes = [vp, i, i, i, i, i, vp, i, vp, i, vp, i, vp, ct.c_size_t, vp]
+    L.fmm_run.restype = i
+    L.fmm_host_run.argtypes = [vp, i, i, i, i, dp, i, dp, i, dp, i]
+    L.fmm_host_run.restype = i
     L.fmm_gemm.argtypes = [i, i, i, i, vp, i, vp, i, vp, i, vp]
     L.fmm_gemm.restype = i
     L.fmm_ws_status.argtypes = [vp, ct.POINTER(i)]
@@ -62,12 +71,16 @@ def _declare(L, has_job: bool = False):
     L.bl_finalize.argtypes = []
     L.bl_finalize.restype = None
     if has_job:
-        L.bl_dgemm_strassen_abc.argtypes = [i, i, i, dp, i, dp, i, dp, i]
-        L.bl_dgemm_strassen_abc.restype = None
-        L.fmm_job_abc.argtypes = [i, i, i, vp, i, vp, i, vp, i, vp, ct.c_size_t, vp]
-        L.fmm_job_abc.restype = i
         L.fmm_job_plan.argtypes = []
         L.fmm_job_plan.restype = vp
+        L.fmm_job_pack.argtypes = []
+        L.fmm_job_pack.restype = i
+        pack = {v: k for k, v in PACKS.items()}[L.fmm_job_pack()]
+        getattr(L, f"bl_dgemm_strassen_{pack}").argtypes = [i, i, i, dp, i, dp, i, dp, i]
+        getattr(L, f"bl_dgemm_strassen_{pack}").restype = None
+        getattr(L, f"fmm_job_{pack}").argtypes = [i, i, i, vp, i, vp, i, vp, i, vp, ct.c_size_t, vp]
+        getattr(L, f"fmm_job_{pack}").restype = i
+        L.job_pack = pack
     return L
 
 
@@ -161,8 +174,19 @@ class Plan:
         except Exception:
             pass
 
-    def workspace_bytes(self, m: int, n: int, k: int) -> int:
-        return int(self._lib.fmm_workspace_bytes(self.handle, m, n, k))
+    def workspace_bytes(self, m: int, n: int, k: int, pack: str = "abc") -> int:
+        if pack == "abc":
+            return int(self._lib.fmm_workspace_bytes(self.handle, m, n, k))
+        return int(self._lib.fmm_workspace_bytes_pack(self.handle, PACKS[pack], m, n, k))
+
+    def run(self, A, B, C, pack: str = "abc", ws=None, stream=None) -> None:
+        """C += A @ B on the GPU by the FMM of pack type `pack` ('abc', 'ab'
+        or 'naive'; FP64 device tensors)."""
+        m, n, k = _check_operands(A, B, C)
+        wptr, wbytes = (None, 0) if ws is None else (ws.data_ptr(), ws.numel() * ws.element_size())
+        check(self._lib.fmm_run(self.handle, PACKS[pack], FMM_F64, m, n, k, A.data_ptr(), _ld(A),
+                                B.data_ptr(), _ld(B), C.data_ptr(), _ld(C), wptr, wbytes,
+                                _stream_ptr(stream)), f"fmm_run({pack})")
 
     def abc(self, A, B, C, ws=None, stream=None) -> None:
         """C += A @ B on the GPU by the ABC FMM (FP64, device tensors)."""
@@ -172,17 +196,20 @@ class Plan:
                                 _ld(B), C.data_ptr(), _ld(C), wptr, wbytes, _stream_ptr(stream)),
               "fmm_abc")
 
-    def host_abc(self, A: np.ndarray, B: np.ndarray, C: np.ndarray) -> None:
+    def host_run(self, A: np.ndarray, B: np.ndarray, C: np.ndarray, pack: str = "abc") -> None:
         """Host-pointer drop-in on Fortran-ordered float64 numpy arrays."""
         dp = ct.POINTER(ct.c_double)
         m, k = A.shape
         n = B.shape[1]
         for X in (A, B, C):
             if X.dtype != np.float64 or not X.flags.f_contiguous:
-                raise ValueError("host_abc needs Fortran-ordered float64 arrays")
-        check(self._lib.fmm_host_abc(self.handle, m, n, k, A.ctypes.data_as(dp), max(m, 1),
-                                     B.ctypes.data_as(dp), max(k, 1), C.ctypes.data_as(dp),
-                                     max(m, 1)), "fmm_host_abc")
+                raise ValueError("host entries need Fortran-ordered float64 arrays")
+        check(self._lib.fmm_host_run(self.handle, PACKS[pack], m, n, k, A.ctypes.data_as(dp),
+                                     max(m, 1), B.ctypes.data_as(dp), max(k, 1),
+                                     C.ctypes.data_as(dp), max(m, 1)), f"fmm_host_run({pack})")
+
+    def host_abc(self, A: np.ndarray, B: np.ndarray, C: np.ndarray) -> None:
+        self.host_run(A, B, C, "abc")
 
 
 def gemm(A, B, C, stream=None, library=None) -> None:

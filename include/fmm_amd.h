@@ -32,6 +32,18 @@ extern "C" {
 /* Element types of A and B (C is FP64 for FMM_F64, FP32 otherwise). */
 #define FMM_F64  0
 
+/* Pack types (the reference's abc / ab / naive generators,
+ * meta/abc_gen_mix.py, meta/ab_gen_mix.py, meta/naive_gen_mix.py):
+ *   ABC    operand sums formed while packing, each product updates its C
+ *          blocks directly;
+ *   AB     operand sums formed while packing, product r into a workspace
+ *          block M_r, then C_j += w_j M_r;
+ *   NAIVE  S_r, T_r summed into workspaces, M_r = S_r T_r, then C_j += w_j M_r.
+ * The three apply the same C updates in the same order and agree bit for bit. */
+#define FMM_PACK_ABC    0
+#define FMM_PACK_AB     1
+#define FMM_PACK_NAIVE  2
+
 typedef struct fmm_plan_s *fmm_plan_t;
 
 /* ---- plans ---------------------------------------------------------------
@@ -50,6 +62,9 @@ int fmm_plan_info(fmm_plan_t plan, int *Mx, int *Kx, int *Nx, int *rank);
 /* Device workspace (bytes) fmm_abc needs for this shape: a few ints of
  * product-order counters; 0 when no ordering is needed. */
 size_t fmm_workspace_bytes(fmm_plan_t plan, int m, int n, int k);
+/* The same for any pack type: AB needs M (R [+ Mx*Nx] leaf blocks ms x ns),
+ * NAIVE also S (R blocks ms x ks) and T (R blocks ks x ns). */
+size_t fmm_workspace_bytes_pack(fmm_plan_t plan, int pack, int m, int n, int k);
 
 /* ---- device-pointer entries (stream ordered, reentrant) -------------------
  * A, B, C are device pointers; `stream` is a hipStream_t (NULL = default).
@@ -61,6 +76,20 @@ size_t fmm_workspace_bytes(fmm_plan_t plan, int m, int n, int k);
 int fmm_abc(fmm_plan_t plan, int dtype, int m, int n, int k,
             const void *A, int lda, const void *B, int ldb, void *C, int ldc,
             void *ws, size_t ws_bytes, void *stream);
+
+/* Any pack type; fmm_abc, fmm_ab and fmm_naive are this with `pack` fixed.
+ * fmm_ab replaces bl_dgemm_strassen_ab (meta/ab_gen_mix.py:341-372),
+ * fmm_naive bl_dgemm_strassen_naive (meta/naive_gen_mix.py:244-361).  ws of
+ * at least fmm_workspace_bytes_pack bytes, or NULL (allocated on `stream`). */
+int fmm_run(fmm_plan_t plan, int pack, int dtype, int m, int n, int k,
+            const void *A, int lda, const void *B, int ldb, void *C, int ldc,
+            void *ws, size_t ws_bytes, void *stream);
+int fmm_ab(fmm_plan_t plan, int dtype, int m, int n, int k,
+           const void *A, int lda, const void *B, int ldb, void *C, int ldc,
+           void *ws, size_t ws_bytes, void *stream);
+int fmm_naive(fmm_plan_t plan, int dtype, int m, int n, int k,
+              const void *A, int lda, const void *B, int ldb, void *C, int ldc,
+              void *ws, size_t ws_bytes, void *stream);
 
 /* Plain GEMM C += A*B on the same kernel (identity plan).
  * Replaces: bl_dgemm (common/dgemm/my_dgemm.c:115-210). */
@@ -75,6 +104,10 @@ int fmm_ws_status(const void *ws, int *flags);
  * fmm_host_abc replaces bl_dgemm_strassen_abc for a run-time plan. */
 int fmm_host_abc(fmm_plan_t plan, int m, int n, int k, const double *A, int lda,
                  const double *B, int ldb, double *C, int ldc);
+/* ... and of any pack type (bl_dgemm_strassen_{abc,ab,naive}). */
+int fmm_host_run(fmm_plan_t plan, int pack, int m, int n, int k,
+                 const double *A, int lda, const double *B, int ldb,
+                 double *C, int ldc);
 
 /* The reference's plain entries, same signatures and C += A*B semantics:
  * common/include/bl_dgemm.h:60-70 (bl_dgemm), :160-170 (bl_dgemm_ref),
@@ -87,17 +120,30 @@ void bl_dgemm_ref(int m, int n, int k, double *XA, int lda, double *XB,
 void bl_finalize(void);
 
 /* ---- per-job libraries only (lib<job>.so from control.py) -----------------
- * The reference's generated entries (meta/abc_gen_mix.py:448,
- * meta/ab_gen_mix.py, meta/naive_gen_mix.py; declared in
- * common/include/bl_dgemm.h:96-130), host pointers, the job's own algorithm: */
+ * A job library exports the entries of its own pack type: the reference's
+ * generated entry (meta/abc_gen_mix.py:448, meta/ab_gen_mix.py:341,
+ * meta/naive_gen_mix.py:244; declared in common/include/bl_dgemm.h:96-130),
+ * host pointers, the job's own algorithm: */
 void bl_dgemm_strassen_abc(int m, int n, int k, double *XA, int lda,
                            double *XB, int ldb, double *XC, int ldc);
-/* and the device-pointer form of the same job (same contract as fmm_abc): */
+void bl_dgemm_strassen_ab(int m, int n, int k, double *XA, int lda,
+                          double *XB, int ldb, double *XC, int ldc);
+void bl_dgemm_strassen_naive(int m, int n, int k, double *XA, int lda,
+                             double *XB, int ldb, double *XC, int ldc);
+/* and the device-pointer form of the same job (the contract of fmm_run): */
 int fmm_job_abc(int m, int n, int k, const double *A, int lda, const double *B,
                 int ldb, double *C, int ldc, void *ws, size_t ws_bytes,
                 void *stream);
-/* the job's plan, for the run-time entries above */
+int fmm_job_ab(int m, int n, int k, const double *A, int lda, const double *B,
+               int ldb, double *C, int ldc, void *ws, size_t ws_bytes,
+               void *stream);
+int fmm_job_naive(int m, int n, int k, const double *A, int lda,
+                  const double *B, int ldb, double *C, int ldc, void *ws,
+                  size_t ws_bytes, void *stream);
+/* every job library: the job's plan (for the run-time entries above) and its
+ * pack type (FMM_PACK_*) */
 fmm_plan_t fmm_job_plan(void);
+int fmm_job_pack(void);
 
 #ifdef __cplusplus
 }

@@ -9,7 +9,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "fmm_amd.h")
-JOB_ONLY = {"bl_dgemm_strassen_abc", "fmm_job_abc", "fmm_job_plan"}
+JOB_ONLY = {"bl_dgemm_strassen_abc", "bl_dgemm_strassen_ab", "bl_dgemm_strassen_naive", "fmm_job_abc",
+            "fmm_job_ab", "fmm_job_naive", "fmm_job_plan", "fmm_job_pack"}
 
 
 def declared():

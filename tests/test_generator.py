@@ -36,14 +36,20 @@ def test_emitted_source(tmp_path, fmm):
     assert len(eqs) == 7
 
 
-@pytest.mark.parametrize("job", ["222-1_abc", "222-1_323-1_abc", "333-1_abc"])
+@pytest.mark.parametrize("job", ["222-1_abc", "222-1_323-1_abc", "333-1_abc", "222-1_ab", "333-1_naive"])
 def test_built_job_exports(job):
     lib = os.path.join(JOBS, job, "lib", f"lib{job}.so")
     if not os.path.exists(lib):
         pytest.skip("jobs not built (run __graft_entry__.build())")
+    pack = job.rsplit("_", 1)[1]
     L = ct.CDLL(lib)
-    for sym in ("bl_dgemm_strassen_abc", "fmm_job_abc", "fmm_job_plan", "fmm_abc", "bl_dgemm"):
+    for sym in (f"bl_dgemm_strassen_{pack}", f"fmm_job_{pack}", "fmm_job_plan", "fmm_job_pack", "fmm_run",
+                "fmm_abc", "bl_dgemm"):
         assert hasattr(L, sym)
+    for other in {"abc", "ab", "naive"} - {pack}:
+        assert not hasattr(L, f"bl_dgemm_strassen_{other}")
+    L.fmm_job_pack.restype = ct.c_int
+    assert L.fmm_job_pack() == {"abc": 0, "ab": 1, "naive": 2}[pack]
     assert os.access(os.path.join(JOBS, job, "test", f"test_{job}_st.x"), os.X_OK)
     assert os.access(os.path.join(JOBS, job, "test", f"test_{job}_mt.x"), os.X_OK)
 
@@ -51,7 +57,9 @@ def test_built_job_exports(job):
 @pytest.mark.gpu
 @pytest.mark.parametrize("job,spec", [("222-1_abc", (["222"], [1])),
                                       ("222-1_323-1_abc", (["222", "323"], [1, 1])),
-                                      ("333-1_abc", (["333"], [1]))])
+                                      ("333-1_abc", (["333"], [1])),
+                                      ("222-1_ab", (["222"], [1])),
+                                      ("333-1_naive", (["333"], [1]))])
 def test_job_library_host_dropin(job, spec, fmm, rt):
     import torch
     if not torch.cuda.is_available():
@@ -68,14 +76,15 @@ def test_job_library_host_dropin(job, spec, fmm, rt):
     Cp = np.zeros((ldc, n + 4), order="F")
     Cp[:m, :n] = C
     dp = ct.POINTER(ct.c_double)
-    L.bl_dgemm_strassen_abc(m, n, k, A.ctypes.data_as(dp), m, B.ctypes.data_as(dp), k, Cp.ctypes.data_as(dp), ldc)
+    entry = getattr(L, f"bl_dgemm_strassen_{L.job_pack}")
+    entry(m, n, k, A.ctypes.data_as(dp), m, B.ctypes.data_as(dp), k, Cp.ctypes.data_as(dp), ldc)
     assert np.array_equal(Cp[:m, :n], want)
     assert not Cp[m:, :].any() and not Cp[:, n:].any()   # nothing outside m x n written
 
 
 @pytest.mark.gpu
-def test_job_driver_prints_reference_line():
-    job = "222-1_abc"
+@pytest.mark.parametrize("job", ["222-1_abc", "333-1_naive"])
+def test_job_driver_prints_reference_line(job):
     exe = os.path.join(JOBS, job, "test", f"test_{job}_mt.x")
     res = subprocess.run([exe, "1000", "1000", "1000"], capture_output=True, text=True, timeout=120,
                          env=dict(os.environ, FMM_SEED="1"))

@@ -1,0 +1,151 @@
+"""GPU parity of the AB and Naive pack types (FP64) against the oracle and
+against the ABC path.
+
+The three pack types apply the same C updates in the same order with the
+same arithmetic (operand sums s = u0*x0, s = fma(u_t, x_t, s); products over
+the whole depth in one accumulator; C_j = fma(w_j, M_r, C_j) in product
+order, then the k-fringe products), so on any input their results must be
+bitwise equal to ABC's.  Integer known-answer cases are also checked
+bit-exact against numpy and against the oracle's restatements of the
+reference's ab / naive generators.
+"""
+import numpy as np
+import pytest
+
+from conftest import int_mats
+from tolerance import norm_err, tol_for
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [(["222"], [1]), (["222"], [2]), (["222", "323"], [1, 1]), (["333"], [1]),
+         (["232"], [1]), (["323"], [1])]
+IDS = ["222L1", "222L2", "222x323", "333L1", "232L1", "323L1"]
+PACKS = ["ab", "naive"]
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def to_dev(torch, rt, X, ld=None):
+    t = rt.colmajor(X.shape[0], X.shape[1], ld=ld, device="cuda")
+    t.copy_(torch.from_numpy(np.ascontiguousarray(X)))
+    return t
+
+
+def run(torch, rt, plan, pack, A, B, C, ldpad=0, ws=None):
+    m, k = A.shape
+    dA = to_dev(torch, rt, A, ld=m + ldpad if ldpad else None)
+    dB = to_dev(torch, rt, B, ld=k + ldpad if ldpad else None)
+    dC = to_dev(torch, rt, C, ld=m + ldpad if ldpad else None)
+    print(f"pack={pack} m={m} n={B.shape[1]} k={k} ldpad={ldpad}", flush=True)
+    plan.run(dA, dB, dC, pack, ws=ws)
+    torch.cuda.synchronize()
+    return dC.cpu().numpy()
+
+
+@pytest.mark.parametrize("pack", PACKS)
+@pytest.mark.parametrize("spec", SPECS, ids=IDS)
+def test_pack_kat_bit_exact(torch, fmm, rt, oracle, spec, pack):
+    alg = fmm.build_algorithm(*spec)
+    plan = rt.Plan(alg)
+    Mx, Kx, Nx = alg.dims
+    rng = np.random.default_rng(15)
+    for (m, n, k) in [(Mx * 8 * 20, Nx * 4 * 40, Kx * 50),             # no fringe, >1 tile
+                      (Mx * 8 * 20 + 7, Nx * 4 * 33 + 3, Kx * 37 + 1),   # all fringes
+                      (Mx * 8 - 3, 50, 20),                              # no FMM core
+                      (Mx * 8 * 3, Nx * 4 * 3, max(Kx - 1, 1))]:         # depth < Kx
+        A, B, C = int_mats(rng, m, n, k)
+        want = C + A @ B
+        got = run(torch, rt, plan, pack, A, B, C)
+        assert np.array_equal(got, want), (spec, pack, m, n, k, float(np.abs(got - want).max()))
+        ref = C.copy(order="F")
+        oracle.fmm(alg, A, B, ref, pack)
+        assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("spec", SPECS, ids=IDS)
+def test_packs_bitwise_equal_abc(torch, fmm, rt, spec):
+    alg = fmm.build_algorithm(*spec)
+    plan = rt.Plan(alg)
+    Mx, Kx, Nx = alg.dims
+    rng = np.random.default_rng(16)
+    for (m, n, k) in [(Mx * 8 * 12 + 5, Nx * 4 * 24 + 2, Kx * 96 + 1),
+                      (Mx * 8 * 17, Nx * 4 * 35, Kx * 71)]:
+        A = rng.uniform(-1, 1, (m, k))
+        B = rng.uniform(-1, 1, (k, n))
+        C = rng.uniform(-1, 1, (m, n))
+        base = run(torch, rt, plan, "abc", A, B, C)
+        assert norm_err(base, A, B, C) <= tol_for(len(alg.levels))
+        for pack in PACKS:
+            got = run(torch, rt, plan, pack, A, B, C)
+            assert np.array_equal(got, base), (spec, pack, m, n, k, float(np.abs(got - base).max()))
+
+
+@pytest.mark.parametrize("pack", PACKS)
+def test_pack_strided_leading_dims(torch, fmm, rt, pack):
+    """Odd leading dimensions and odd leaf depth take the element-wise paths
+    of the sum, product and M_Add kernels."""
+    alg = fmm.build_algorithm(["222"], [1])
+    plan = rt.Plan(alg)
+    rng = np.random.default_rng(17)
+    A, B, C = int_mats(rng, 203, 161, 99)
+    for pad in (1, 3, 8):
+        got = run(torch, rt, plan, pack, A, B, C, ldpad=pad)
+        assert np.array_equal(got, C + A @ B), pad
+
+
+@pytest.mark.parametrize("pack", PACKS)
+def test_pack_workspace(torch, fmm, rt, pack):
+    alg = fmm.build_algorithm(["333"], [1])
+    plan = rt.Plan(alg)
+    m, n, k = 720, 600, 301
+    nbytes = plan.workspace_bytes(m, n, k, pack)
+    ms, ks, ns = 240, 100, 200
+    Mblocks = alg.rank + 9            # products + k-fringe products
+    want = -(-8 * ms * ns * Mblocks // 256) * 256
+    if pack == "naive":
+        want += -(-8 * ms * ks * alg.rank // 256) * 256 + -(-8 * ks * ns * alg.rank // 256) * 256
+    assert nbytes == want
+    rng = np.random.default_rng(18)
+    A, B, C = int_mats(rng, m, n, k)
+    ws = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    for _ in range(2):   # reuse: nothing depends on the workspace's old contents
+        got = run(torch, rt, plan, pack, A, B, C, ws=ws)
+        assert np.array_equal(got, C + A @ B)
+    small = torch.empty(nbytes // 8 - 1, dtype=torch.float64, device="cuda")
+    with pytest.raises(rt.FmmError):
+        run(torch, rt, plan, pack, A, B, C, ws=small)
+
+
+@pytest.mark.parametrize("pack", PACKS)
+def test_pack_host_dropin(torch, fmm, rt, pack):
+    alg = fmm.build_algorithm(["232"], [1])
+    plan = rt.Plan(alg)
+    rng = np.random.default_rng(19)
+    A, B, C = int_mats(rng, 150, 90, 61)
+    got = C.copy(order="F")
+    plan.host_run(A, B, got, pack)
+    assert np.array_equal(got, C + A @ B)
+
+
+def test_naive_config1_strassen_2048(torch, fmm, rt, oracle):
+    """Strassen L1 FP64 at m=n=k=2048, A, B ~ U[0,1), C = 0: Naive equals ABC
+    bitwise and both are within the one-level bound of the exact product."""
+    alg = fmm.build_algorithm(["222"], [1])
+    plan = rt.Plan(alg)
+    rng = np.random.default_rng(2048)
+    m = n = k = 2048
+    A = np.asfortranarray(rng.random((m, k)))
+    B = np.asfortranarray(rng.random((k, n)))
+    C = np.zeros((m, n), order="F")
+    got = run(torch, rt, plan, "naive", A, B, C)
+    assert np.array_equal(got, run(torch, rt, plan, "abc", A, B, C))
+    assert norm_err(got, A, B, C) <= tol_for(1)
+    ref = C.copy(order="F")
+    oracle.fmm(alg, A, B, ref, "naive")
+    assert float(np.abs(got - ref).max()) <= 2 * tol_for(1) * 2.0 ** -53 * k
