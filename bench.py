@@ -28,6 +28,9 @@ WORKLOADS = {
     "333L1_rankk14400x480": (["333"], [1], 14400, 14400, 480),
     "222x323_rankk14400x480": (["222", "323"], [1, 1], 14400, 14400, 480),
     "222L1_sq2048": (["222"], [1], 2048, 2048, 2048),
+    "222L1_sq9600": (["222"], [1], 9600, 9600, 9600),
+    "222L2_sq9600": (["222"], [2], 9600, 9600, 9600),
+    "222L1_rankk14400x480": (["222"], [1], 14400, 14400, 480),
 }
 DEFAULT = "333L1_sq9600"
 PEAK_F64_TFLOPS = 78.6

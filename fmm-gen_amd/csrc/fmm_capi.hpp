@@ -20,9 +20,11 @@ inline bool bad_dims(int m, int n, int k, int lda, int ldb, int ldc) {
   return false;
 }
 
-inline int run_abc(Plan* P, int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C,
-                   int ldc, void* ws, size_t ws_bytes, hipStream_t s) {
+inline int run_abc(Plan* P, int dtype, int m, int n, int k, const void* A, int lda, const void* B, int ldb,
+                   void* C, int ldc, void* ws, size_t ws_bytes, hipStream_t s) {
   if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
+  KGeo G;
+  if (!dtype_geo(dtype, &G)) return kEUnsupported;
   if (m == 0 || n == 0 || k == 0) return kOK;
   Prod* dp;
   Term* dt;
@@ -33,7 +35,7 @@ inline int run_abc(Plan* P, int m, int n, int k, const double* A, int lda, const
   rc = device_tables(identity_plan(), &ip, &it);
   if (rc != kOK) return rc;
   JobList JL;
-  add_fmm_jobs(JL, *P, dp, dt, ip, it, m, n, k, A, lda, B, ldb, C, ldc);
+  add_fmm_jobs(JL, G, *P, dp, dt, ip, it, m, n, k, A, lda, B, ldb, C, ldc);
   const size_t need = ws_ints(JL) * sizeof(int);
   int* w = static_cast<int*>(ws);
   bool own = false;
@@ -43,21 +45,22 @@ inline int run_abc(Plan* P, int m, int n, int k, const double* A, int lda, const
   } else if (need > ws_bytes && w) {
     return kEInval;
   }
-  rc = launch_jobs(JL, P->max_nnz, w, s);
+  rc = launch_jobs_dtype(dtype, JL, P->max_nnz, w, s);
   if (own) (void)hipFreeAsync(w, s);
   return rc;
 }
 
 // One FMM call of pack type `pack`.  AB / Naive need the workspace of
 // pack_layout; a shape without an FMM core runs the ABC path (plain GEMMs).
-inline int run_fmm(Plan* P, int pack, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
-                   double* C, int ldc, void* ws, size_t ws_bytes, hipStream_t s) {
+inline int run_fmm(Plan* P, int pack, int dtype, int m, int n, int k, const void* A, int lda, const void* B,
+                   int ldb, void* C, int ldc, void* ws, size_t ws_bytes, hipStream_t s) {
   if (pack != kPackABC && pack != kPackAB && pack != kPackNaive) return kEInval;
   if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
+  if (pack != kPackABC && dtype != kF64) return kEUnsupported;
   if (m == 0 || n == 0 || k == 0) return kOK;
   PackLayout Lo;
   if (pack == kPackABC || !pack_layout(*P, pack, m, n, k, &Lo))
-    return run_abc(P, m, n, k, A, lda, B, ldb, C, ldc, ws, ws_bytes, s);
+    return run_abc(P, dtype, m, n, k, A, lda, B, ldb, C, ldc, ws, ws_bytes, s);
   void* w = ws;
   bool own = false;
   if (!w) {
@@ -66,7 +69,8 @@ inline int run_fmm(Plan* P, int pack, int m, int n, int k, const double* A, int 
   } else if (ws_bytes < Lo.bytes) {
     return kEInval;
   }
-  int rc = launch_pack(P, pack, Lo, m, n, k, A, lda, B, ldb, C, ldc, w, s);
+  int rc = launch_pack(P, pack, Lo, m, n, k, static_cast<const double*>(A), lda, static_cast<const double*>(B), ldb,
+                       static_cast<double*>(C), ldc, w, s);
   if (own) (void)hipFreeAsync(w, s);
   return rc;
 }
@@ -81,17 +85,19 @@ inline size_t workspace_bytes(Plan* P, int pack, int m, int n, int k) {
   return ws_ints(JL) * sizeof(int);
 }
 
-inline int run_gemm(int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
+inline int run_gemm(int dtype, int m, int n, int k, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
                     hipStream_t s) {
   if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
+  KGeo G;
+  if (!dtype_geo(dtype, &G)) return kEUnsupported;
   if (m == 0 || n == 0 || k == 0) return kOK;
   Prod* ip;
   Term* it;
   int rc = device_tables(identity_plan(), &ip, &it);
   if (rc != kOK) return rc;
   JobList JL;
-  add_gemm_job(JL, ip, it, m, n, k, A, lda, B, ldb, C, ldc);
-  return launch_jobs(JL, 1, nullptr, s);
+  add_gemm_job(JL, G, ip, it, m, n, k, A, lda, B, ldb, C, ldc);
+  return launch_jobs_dtype(dtype, JL, 1, nullptr, s);
 }
 
 // Staging buffers of the host-pointer drop-ins (grow-only, one set per
@@ -206,9 +212,8 @@ size_t fmm_workspace_bytes_pack(fmm_plan_t plan, int pack, int m, int n, int k) 
 int fmm_run(fmm_plan_t plan, int pack, int dtype, int m, int n, int k, const void* A, int lda, const void* B,
             int ldb, void* C, int ldc, void* ws, size_t ws_bytes, void* stream) {
   if (!plan) return fmm::kEInval;
-  if (dtype != FMM_F64) return fmm::kEUnsupported;
-  return fmm::run_fmm(&plan->p, pack, m, n, k, static_cast<const double*>(A), lda, static_cast<const double*>(B),
-                      ldb, static_cast<double*>(C), ldc, ws, ws_bytes, static_cast<hipStream_t>(stream));
+  return fmm::run_fmm(&plan->p, pack, dtype, m, n, k, A, lda, B, ldb, C, ldc, ws, ws_bytes,
+                      static_cast<hipStream_t>(stream));
 }
 
 int fmm_abc(fmm_plan_t plan, int dtype, int m, int n, int k, const void* A, int lda, const void* B, int ldb, void* C,
@@ -228,9 +233,7 @@ int fmm_naive(fmm_plan_t plan, int dtype, int m, int n, int k, const void* A, in
 
 int fmm_gemm(int dtype, int m, int n, int k, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
              void* stream) {
-  if (dtype != FMM_F64) return fmm::kEUnsupported;
-  return fmm::run_gemm(m, n, k, static_cast<const double*>(A), lda, static_cast<const double*>(B), ldb,
-                       static_cast<double*>(C), ldc, static_cast<hipStream_t>(stream));
+  return fmm::run_gemm(dtype, m, n, k, A, lda, B, ldb, C, ldc, static_cast<hipStream_t>(stream));
 }
 
 int fmm_ws_status(const void* ws, int* flags) {
@@ -248,7 +251,7 @@ int fmm_host_run(fmm_plan_t plan, int pack, int m, int n, int k, const double* A
   if (!plan) return fmm::kEInval;
   return fmm::host_call(m, n, k, A, lda, B, ldb, C, ldc,
                         [&](const double* a, int la, const double* b, int lb, double* c, int lc) {
-                          int rc = fmm::run_fmm(&plan->p, pack, m, n, k, a, la, b, lb, c, lc, nullptr, 0, nullptr);
+                          int rc = fmm::run_fmm(&plan->p, pack, fmm::kF64, m, n, k, a, la, b, lb, c, lc, nullptr, 0, nullptr);
                           if (rc == fmm::kOK && hipDeviceSynchronize() != hipSuccess) rc = fmm::kEHip;
                           return rc;
                         });
@@ -262,7 +265,7 @@ int fmm_host_abc(fmm_plan_t plan, int m, int n, int k, const double* A, int lda,
 void bl_dgemm(int m, int n, int k, double* XA, int lda, double* XB, int ldb, double* XC, int ldc) {
   int rc = fmm::host_call(m, n, k, XA, lda, XB, ldb, XC, ldc,
                           [&](const double* a, int la, const double* b, int lb, double* c, int lc) {
-                            int r = fmm::run_gemm(m, n, k, a, la, b, lb, c, lc, nullptr);
+                            int r = fmm::run_gemm(fmm::kF64, m, n, k, a, la, b, lb, c, lc, nullptr);
                             if (r == fmm::kOK && hipDeviceSynchronize() != hipSuccess) r = fmm::kEHip;
                             return r;
                           });

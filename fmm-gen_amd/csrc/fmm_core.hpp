@@ -57,10 +57,13 @@ struct Prod {
   int a0, na, b0, nb, c0, nc, fringe, pad;
 };
 
+// Operand pointers are untyped: the kernel that runs the job knows the
+// element types (FP64 here; FP32 / BF16 / FP16 inputs with FP32 C in the
+// low-precision kernels).
 struct Job {
-  const double* A;
-  const double* B;
-  double* C;
+  const void* A;
+  const void* B;
+  void* C;
   long long lda, ldb, ldc;
   int ms, ns, ks, kr;       // leaf block and k-fringe depth
   int tiles_m, tiles_n;     // C tiles over one leaf block
@@ -210,10 +213,10 @@ struct UnitGeo {
 };
 
 __device__ __forceinline__ gcdouble* a_block(const Job& J, const Term& t) {
-  return gptr(J.A) + (long long)t.bi * J.ms + (long long)t.bj * J.ks * J.lda;
+  return gptr(static_cast<const double*>(J.A)) + (long long)t.bi * J.ms + (long long)t.bj * J.ks * J.lda;
 }
 __device__ __forceinline__ gcdouble* b_block(const Job& J, const Term& t) {
-  return gptr(J.B) + (long long)t.bi * J.ks + (long long)t.bj * J.ns * J.ldb;
+  return gptr(static_cast<const double*>(J.B)) + (long long)t.bi * J.ks + (long long)t.bj * J.ns * J.ldb;
 }
 
 // Uniform (wave-invariant) copies of values the compiler cannot prove
@@ -505,7 +508,7 @@ __device__ __forceinline__ void scatter_c(const Job& J, const Prod& P, int row0,
   for (int t = 0; t < P.nc; ++t) {
     const Term tm = term_at(J, P.c0 + t);
     const double w = tm.coef;
-    gdouble* cb = gptr(J.C) + (long long)tm.bi * J.ms + (long long)tm.bj * J.ns * J.ldc;
+    gdouble* cb = gptr(static_cast<double*>(J.C)) + (long long)tm.bi * J.ms + (long long)tm.bj * J.ns * J.ldc;
 #pragma unroll
     for (int mi = 0; mi < FM; ++mi) {
       const int row = row0 + wm * WTM + mi * 16 + l16;

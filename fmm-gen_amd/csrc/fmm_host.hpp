@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "fmm_core.hpp"
+#include "fmm_lowp.hpp"
 
 namespace fmm {
 
@@ -314,27 +315,46 @@ inline bool probe_xcd_round_robin() {
   return ok;
 }
 
-template <int BK, int CH>
-inline int dev_info(DevInfo* out) {
+// Launch properties of `kernel` (nt threads per workgroup) on the current
+// device, cached per (device, kernel).
+inline int dev_info(const void* kernel, int nt, DevInfo* out) {
+  struct Entry {
+    int dev;
+    const void* kernel;
+    DevInfo d;
+  };
   static std::mutex mu;
-  static std::vector<DevInfo> cache;
+  static std::vector<Entry> cache;
+  static std::vector<int> rr;   // per device: -1 unknown, 0/1 probed
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return kEHip;
   std::lock_guard<std::mutex> g(mu);
-  if ((int)cache.size() <= dev) cache.resize(dev + 1);
-  DevInfo& d = cache[dev];
-  if (d.cus == 0) {
-    int cus = 0, khz = 0, nb = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return kEHip;
-    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, abc_f64_kernel<BK, CH>, NT, 0) != hipSuccess) return kEHip;
-    d.cus = cus;
-    d.blocks_per_cu = std::max(nb, 1);
-    d.xcd_round_robin = probe_xcd_round_robin();
-    d.spin_ticks = (long long)khz * 1000LL * 5LL;   // 5 s
-  }
-  *out = d;
+  for (const Entry& e : cache)
+    if (e.dev == dev && e.kernel == kernel) {
+      *out = e.d;
+      return kOK;
+    }
+  int cus = 0, khz = 0, nb = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return kEHip;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, nt, 0) != hipSuccess) return kEHip;
+  if ((int)rr.size() <= dev) rr.resize(dev + 1, -1);
+  if (rr[dev] < 0) rr[dev] = probe_xcd_round_robin() ? 1 : 0;
+  Entry e;
+  e.dev = dev;
+  e.kernel = kernel;
+  e.d.cus = cus;
+  e.d.blocks_per_cu = std::max(nb, 1);
+  e.d.xcd_round_robin = rr[dev] == 1;
+  e.d.spin_ticks = (long long)khz * 1000LL * 5LL;   // 5 s
+  cache.push_back(e);
+  *out = e.d;
   return kOK;
+}
+
+template <int BK, int CH>
+inline int dev_info(DevInfo* out) {
+  return dev_info(reinterpret_cast<const void*>(&abc_f64_kernel<BK, CH>), NT, out);
 }
 
 // --- job construction -------------------------------------------------------
@@ -347,17 +367,43 @@ struct JobList {
 
 inline int tiles(int n, int t) { return (n + t - 1) / t; }
 
-inline int vec_flags(const void* A, long long lda, const void* B, long long ldb, int ks_even, const void* C,
-                     long long ldc) {
+// Geometry of the kernel family that runs a job list: its C tile, the bytes
+// of an A/B and of a C element, and the bytes of its A vector access (B and C
+// vector accesses are 16 bytes in every family).
+struct KGeo {
+  int bm, bn;
+  int sin, sc;
+  int va;
+};
+constexpr KGeo kGeoF64{BM, BN, 8, 8, 16};
+
+// Vector-access flags of a job (bits 0/1/2: A/B/C).  `kblk` is the row
+// offset unit of B blocks (the leaf depth for FMM jobs, 0 for a plain GEMM,
+// whose only block starts at B itself).
+inline int vec_flags(const KGeo& G, const void* A, long long lda, const void* B, long long ldb, long long kblk,
+                     const void* C, long long ldc) {
   int f = 0;
-  if (((uintptr_t)A & 15) == 0 && (lda & 1) == 0) f |= 1;
-  if (((uintptr_t)B & 15) == 0 && (ldb & 1) == 0 && ks_even) f |= 2;
-  if (((uintptr_t)C & 15) == 0 && (ldc & 1) == 0) f |= 4;
+  if (((uintptr_t)A % G.va) == 0 && (lda * G.sin) % G.va == 0) f |= 1;
+  if (((uintptr_t)B & 15) == 0 && (ldb * G.sin) % 16 == 0 && (kblk * G.sin) % 16 == 0) f |= 2;
+  if (((uintptr_t)C & 15) == 0 && (ldc * G.sc) % 16 == 0) f |= 4;
   return f;
 }
 
-inline void add_gemm_job(JobList& L, const Prod* dp, const Term* dt, int m, int n, int k, const double* A,
-                         long long lda, const double* B, long long ldb, double* C, long long ldc) {
+inline int vec_flags(const void* A, long long lda, const void* B, long long ldb, int ks_even, const void* C,
+                     long long ldc) {
+  return vec_flags(kGeoF64, A, lda, B, ldb, ks_even ? 0 : 1, C, ldc);
+}
+
+// Element offset of a matrix pointer.
+inline const void* eoff(const void* p, long long elems, int size) {
+  return p ? static_cast<const char*>(p) + elems * size : nullptr;
+}
+inline void* eoff(void* p, long long elems, int size) {
+  return p ? static_cast<char*>(p) + elems * size : nullptr;
+}
+
+inline void add_gemm_job(JobList& L, const KGeo& G, const Prod* dp, const Term* dt, int m, int n, int k,
+                         const void* A, long long lda, const void* B, long long ldb, void* C, long long ldc) {
   if (m <= 0 || n <= 0 || k <= 0) return;
   Job j{};
   j.A = A;
@@ -370,38 +416,44 @@ inline void add_gemm_job(JobList& L, const Prod* dp, const Term* dt, int m, int 
   j.ns = n;
   j.ks = k;
   j.kr = 0;
-  j.tiles_m = tiles(m, BM);
-  j.tiles_n = tiles(n, BN);
+  j.tiles_m = tiles(m, G.bm);
+  j.tiles_n = tiles(n, G.bn);
   j.nprod = 1;
   j.gsize = 1;
   j.ngroups = 1;
   j.unit_base = L.total_units;
   j.order_base = -1;
-  j.flags = vec_flags(A, lda, B, ldb, 1, C, ldc);
+  j.flags = vec_flags(G, A, lda, B, ldb, 0, C, ldc);
   j.prods = dp;
   j.terms = dt;
   L.total_units += j.tiles_m * j.tiles_n;
   L.jobs.push_back(j);
 }
 
-// Jobs for one FMM call C += A*B (column-major, FP64): the two independent
-// fringe GEMMs first (they are thin and long, so they start early), then the
-// FMM core with the k-fringe products appended (bl_dynamic_peeling,
+inline void add_gemm_job(JobList& L, const Prod* dp, const Term* dt, int m, int n, int k, const double* A,
+                         long long lda, const double* B, long long ldb, double* C, long long ldc) {
+  add_gemm_job(L, kGeoF64, dp, dt, m, n, k, A, lda, B, ldb, C, ldc);
+}
+
+// Jobs for one FMM call C += A*B (column-major): the two independent fringe
+// GEMMs first (they are thin and long, so they start early), then the FMM
+// core with the k-fringe products appended (bl_dynamic_peeling,
 // common/dgemm/bl_dgemm_util.c:316-353, reorganised as jobs of one launch).
-inline void add_fmm_jobs(JobList& L, const Plan& P, const Prod* dp, const Term* dt, const Prod* idp,
-                         const Term* idt, int m, int n, int k, const double* A, long long lda, const double* B,
-                         long long ldb, double* C, long long ldc) {
+inline void add_fmm_jobs(JobList& L, const KGeo& G, const Plan& P, const Prod* dp, const Term* dt, const Prod* idp,
+                         const Term* idt, int m, int n, int k, const void* A, long long lda, const void* B,
+                         long long ldb, void* C, long long ldc) {
   const int md = m - m % (P.Mx * kMR), kd = k - k % P.Kx, nd = n - n % (P.Nx * kNR);
   const int ms = md / P.Mx, ks = kd / P.Kx, ns = nd / P.Nx;
   const int kr = k - kd;
   // (ii) right columns C[:, nd:n] += A * B[:, nd:n]
-  add_gemm_job(L, idp, idt, m, n - nd, k, A, lda, B + (long long)nd * ldb, ldb, C + (long long)nd * ldc, ldc);
+  add_gemm_job(L, G, idp, idt, m, n - nd, k, A, lda, eoff(B, (long long)nd * ldb, G.sin), ldb,
+               eoff(C, (long long)nd * ldc, G.sc), ldc);
   // (iii) bottom rows C[md:m, 0:nd] += A[md:m, :] * B[:, 0:nd]
-  if (nd > 0) add_gemm_job(L, idp, idt, m - md, nd, k, A + md, lda, B, ldb, C + md, ldc);
+  if (nd > 0) add_gemm_job(L, G, idp, idt, m - md, nd, k, eoff(A, md, G.sin), lda, B, ldb, eoff(C, md, G.sc), ldc);
   if (ms <= 0 || ns <= 0) return;
   if (ks <= 0) {
     // no FMM core depth: the k-fringe alone is a plain GEMM on the core block
-    add_gemm_job(L, idp, idt, md, nd, k, A, lda, B, ldb, C, ldc);
+    add_gemm_job(L, G, idp, idt, md, nd, k, A, lda, B, ldb, C, ldc);
     return;
   }
   Job j{};
@@ -415,14 +467,14 @@ inline void add_fmm_jobs(JobList& L, const Plan& P, const Prod* dp, const Term* 
   j.ns = ns;
   j.ks = ks;
   j.kr = kr;
-  j.tiles_m = tiles(ms, BM);
-  j.tiles_n = tiles(ns, BN);
+  j.tiles_m = tiles(ms, G.bm);
+  j.tiles_n = tiles(ns, G.bn);
   j.nprod = P.R + (kr > 0 ? P.Mx * P.Nx : 0);
   j.gsize = 1;
   j.ngroups = j.nprod;
   j.unit_base = L.total_units;
   j.order_base = j.nprod > 1 ? L.counters : -1;
-  j.flags = vec_flags(A, lda, B, ldb, (ks & 1) == 0, C, ldc);
+  j.flags = vec_flags(G, A, lda, B, ldb, ks, C, ldc);
   j.prods = dp;
   j.terms = dt;
   if (j.nprod > 1) {
@@ -431,6 +483,12 @@ inline void add_fmm_jobs(JobList& L, const Plan& P, const Prod* dp, const Term* 
   }
   L.total_units += j.nprod * j.tiles_m * j.tiles_n;
   L.jobs.push_back(j);
+}
+
+inline void add_fmm_jobs(JobList& L, const Plan& P, const Prod* dp, const Term* dt, const Prod* idp,
+                         const Term* idt, int m, int n, int k, const double* A, long long lda, const double* B,
+                         long long ldb, double* C, long long ldc) {
+  add_fmm_jobs(L, kGeoF64, P, dp, dt, idp, idt, m, n, k, A, lda, B, ldb, C, ldc);
 }
 
 // Group consecutive products of each FMM job into units so that the launch
@@ -467,14 +525,14 @@ inline void group_products(JobList& L, int slots) {
 
 inline size_t ws_ints(const JobList& L) { return L.ordered ? (size_t)(kWsCounters + L.counters) : 0; }
 
-// Launch a job list (<= kInlineJobs jobs, passed by value) on `stream`.
-// `ws` must hold ws_ints(L) ints when L.ordered.
-template <int BK, int CH>
-inline int launch_jobs_k(JobList JL, int* ws, hipStream_t stream) {
+// Launch a job list (<= kInlineJobs jobs, passed by value) on `stream` with
+// the persistent kernel `kern` (nt threads per workgroup).  `ws` must hold
+// ws_ints(L) ints when L.ordered.
+inline int launch_jobs_with(void (*kern)(Launch), int nt, JobList JL, int* ws, hipStream_t stream) {
   if (JL.total_units == 0) return kOK;
   if ((int)JL.jobs.size() > kInlineJobs) return kEInval;
   DevInfo di;
-  int rc = dev_info<BK, CH>(&di);
+  int rc = dev_info(reinterpret_cast<const void*>(kern), nt, &di);
   if (rc != kOK) return rc;
   const int slots = di.cus * di.blocks_per_cu;
   const size_t ws_n = ws_ints(JL);   // counters as reserved by the caller
@@ -503,9 +561,14 @@ inline int launch_jobs_k(JobList JL, int* ws, hipStream_t stream) {
     L.nq = kMaxQueues;
     L.local_handover = JL.ordered && di.xcd_round_robin ? 1 : 0;
   }
-  abc_f64_kernel<BK, CH><<<dim3(grid), dim3(NT), 0, stream>>>(L);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(nt), 0, stream, L);
   if (hipGetLastError() != hipSuccess) return kEHip;
   return kOK;
+}
+
+template <int BK, int CH>
+inline int launch_jobs_k(JobList JL, int* ws, hipStream_t stream) {
+  return launch_jobs_with(abc_f64_kernel<BK, CH>, NT, std::move(JL), ws, stream);
 }
 
 // Kernel variant by the largest operand-sum term count of the launch:
@@ -514,6 +577,40 @@ inline int launch_jobs_k(JobList JL, int* ws, hipStream_t stream) {
 inline int launch_jobs(const JobList& JL, int max_terms, int* ws, hipStream_t stream) {
   if (max_terms <= 2) return launch_jobs_k<32, 2>(JL, ws, stream);
   return launch_jobs_k<16, 4>(JL, ws, stream);
+}
+
+// --- element types ------------------------------------------------------------
+//
+// FP64: A, B, C FP64 (abc_f64_kernel).  FP32 / BF16 / FP16: A and B of that
+// type, C FP32, products on FP32 / BF16 / FP16 MFMA with FP32 accumulation.
+// BF16x3: A, B, C FP32, split-BF16 (compensated) products.
+enum DType : int { kF64 = 0, kF32 = 1, kBF16 = 2, kF16 = 3, kBF16x3 = 4 };
+
+template <class C>
+constexpr KGeo lgeo() {
+  return KGeo{C::BM, C::BN, C::IN == kInF32 ? 4 : 2, 4, C::AR * (C::IN == kInF32 ? 4 : 2)};
+}
+
+inline bool dtype_geo(int dtype, KGeo* g) {
+  switch (dtype) {
+    case kF64: *g = kGeoF64; return true;
+    case kF32: *g = lgeo<CfgF32>(); return true;
+    case kBF16: *g = lgeo<CfgBF16>(); return true;
+    case kF16: *g = lgeo<CfgF16>(); return true;
+    case kBF16x3: *g = lgeo<CfgBF16x3>(); return true;
+    default: return false;
+  }
+}
+
+inline int launch_jobs_dtype(int dtype, const JobList& JL, int max_terms, int* ws, hipStream_t stream) {
+  switch (dtype) {
+    case kF64: return launch_jobs(JL, max_terms, ws, stream);
+    case kF32: return launch_jobs_with(abc_lowp_kernel<CfgF32>, CfgF32::NT, JL, ws, stream);
+    case kBF16: return launch_jobs_with(abc_lowp_kernel<CfgBF16>, CfgBF16::NT, JL, ws, stream);
+    case kF16: return launch_jobs_with(abc_lowp_kernel<CfgF16>, CfgF16::NT, JL, ws, stream);
+    case kBF16x3: return launch_jobs_with(abc_lowp_kernel<CfgBF16x3>, CfgBF16x3::NT, JL, ws, stream);
+    default: return kEUnsupported;
+  }
 }
 
 // --- AB and Naive pack types ---------------------------------------------

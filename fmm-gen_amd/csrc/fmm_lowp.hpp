@@ -1,0 +1,671 @@
+// fmm_lowp.hpp -- low-precision members of the ABC kernel family (gfx950).
+//
+// Same jobs, units, product order and C hand-over as the FP64 kernel of
+// fmm_core.hpp; what changes is the arithmetic of a unit:
+//   * A and B hold FP32, BF16 or FP16 elements; C is FP32.
+//   * Operand sums  S = sum_t u_t A_t,  T = sum_t v_t B_t  are formed in FP32
+//     from the loaded elements (s = u0*x0, then s = fma(u_t, x_t, s), the
+//     order of the reference's packing, meta/abc_gen_mix.py:229-251), then
+//     rounded once to the MFMA operand type (round to nearest even).
+//   * Products run on MFMA with FP32 accumulation over the whole leaf depth:
+//       FP32 operands   v_mfma_f32_16x16x4_f32
+//       BF16 operands   v_mfma_f32_16x16x32_bf16
+//       FP16 operands   v_mfma_f32_16x16x32_f16
+//   * Split-BF16 (compensated) products: FP32 inputs, each operand sum split
+//     as S = S_hi + S_lo with S_hi = bf16(S), S_lo = bf16(S - S_hi); a
+//     product is S_hi T_hi + S_hi T_lo + S_lo T_hi (three BF16 MFMAs, FP32
+//     accumulation), about 16 significant bits per operand instead of 8.
+//   * W scatter: C_j = fma(w_j, acc, C_j) in FP32, 16-byte accesses.
+//
+// MFMA register layout (16x16 output, FP32 accumulators): lane l supplies
+// row l%16 of the left operand and column l%16 of the right operand; its
+// k values are k = l/16 (x4 f32) or k = 8*(l/16) + e, e = 0..7 (x32 16-bit);
+// accumulator element v of lane l is D[4*(l/16) + v][l%16].  The left operand
+// is A (rows of C), the right one B, so each lane owns 4 consecutive rows of
+// one C column: one 16-byte read-modify-write of column-major C.
+//
+// LDS layouts.  16-bit operands: As[i][k] and Bs[j][k], k contiguous, row
+// stride BK+8 elements, so a lane's 8 k values are one 16-byte read and the
+// 16 rows read together fall on distinct banks.  A arrives column-major (i
+// contiguous), so the packing step transposes it: a thread loads an AR x 8
+// block (8 loads of AR consecutive rows), and writes AR rows of 8 k.  FP32
+// operands: As[k][i] (row stride BM+16 floats, i contiguous as loaded) and
+// Bs[j][k] (row stride BK+2), each MFMA operand one 4-byte read.
+#pragma once
+#include <type_traits>
+
+#include "fmm_core.hpp"
+
+namespace fmm {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const float gcfloat;
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) const u16 gcu16;
+typedef __attribute__((address_space(1))) const f32x4 gcf32x4;
+typedef __attribute__((address_space(1))) f32x4 gf32x4;
+typedef __attribute__((address_space(1))) const u32x2 gcuint2;
+typedef __attribute__((address_space(1))) const u32x4 gcuint4;
+typedef __attribute__((address_space(1))) const f32x2 gcf32x2;
+
+// Input element kinds and MFMA operand kinds.
+enum : int { kInF32 = 0, kInBF16 = 1, kInF16 = 2 };
+enum : int { kOpF32 = 0, kOpBF16 = 1, kOpF16 = 2 };
+
+template <int IN>
+struct InT {
+  typedef u16 T;   // 16-bit inputs are carried as raw bits
+};
+template <>
+struct InT<kInF32> {
+  typedef float T;
+};
+
+__device__ __forceinline__ float bf16_bits_to_f(unsigned int b) { return __uint_as_float(b << 16); }
+__device__ __forceinline__ float f16_bits_to_f(unsigned int b) {
+  return (float)__builtin_bit_cast(_Float16, (unsigned short)(b & 0xffffu));
+}
+
+template <int IN>
+__device__ __forceinline__ float in_to_f(typename InT<IN>::T x) {
+  if constexpr (IN == kInF32) return x;
+  else if constexpr (IN == kInBF16) return bf16_bits_to_f(x);
+  else return f16_bits_to_f(x);
+}
+
+// Unpack a 32-bit word of two 16-bit inputs (element 0 in the low half).
+template <int IN>
+__device__ __forceinline__ void unpack2(unsigned int w, float& lo, float& hi) {
+  if constexpr (IN == kInBF16) {
+    lo = __uint_as_float(w << 16);
+    hi = __uint_as_float(w & 0xffff0000u);
+  } else {
+    lo = f16_bits_to_f(w);
+    hi = f16_bits_to_f(w >> 16);
+  }
+}
+
+// Kernel configuration.
+template <int IN_, int OP_, bool SPLIT_, int BM_, int BN_, int BK_, int WM_, int WN_, int CH_>
+struct LCfg {
+  static constexpr int IN = IN_, OP = OP_;
+  static constexpr bool SPLIT = SPLIT_;
+  static constexpr int BM = BM_, BN = BN_, BK = BK_;
+  static constexpr int WM = WM_, WN = WN_;            // waves along m and n
+  static constexpr int CH = CH_;                      // operand terms per prefetched chunk
+  static constexpr int NT = 64 * WM * WN;
+  static constexpr int WTM = BM / WM, WTN = BN / WN;  // wave tile
+  static constexpr int FM = WTM / 16, FN = WTN / 16;  // MFMA blocks per wave
+  static constexpr bool H16 = OP != kOpF32;           // 16-bit MFMA operands
+  static constexpr int KM = H16 ? 32 : 4;             // k per MFMA
+  static constexpr int NPL = SPLIT ? 2 : 1;           // operand planes (hi, lo)
+  // A items: AR rows x AK k per item.  B items: BKV consecutive k of one column.
+  static constexpr int AR = H16 ? ((BM * BK) / (8 * NT) >= 4 ? 4 : 2) : 4;
+  static constexpr int AK = H16 ? 8 : 1;
+  static constexpr int BKV = H16 ? 8 : 4;
+  static constexpr int NA = (BM * BK) / (AR * AK * NT);   // A items per thread
+  static constexpr int NB = (BN * BK) / (BKV * NT);       // B items per thread
+  // LDS strides (elements of the operand type)
+  static constexpr int SA = H16 ? BK + 8 : BM + 16;
+  static constexpr int SB = H16 ? BK + 8 : BK + 2;
+  static constexpr int A_ELEMS = H16 ? BM * SA : BK * SA;
+  static constexpr int B_ELEMS = BN * SB;
+  static constexpr int OPSZ = H16 ? 2 : 4;
+  static constexpr int LDS_BYTES = 2 * NPL * (A_ELEMS + B_ELEMS) * OPSZ;
+  static_assert(NA >= 1 && NA * AR * AK * NT == BM * BK, "A item layout");
+  static_assert(NB >= 1 && NB * BKV * NT == BN * BK, "B item layout");
+  static_assert(BK % KM == 0 && WTM % 16 == 0 && WTN % 16 == 0, "tile shape");
+  static_assert(!SPLIT || (IN == kInF32 && OP == kOpBF16), "split products take FP32 inputs on BF16 MFMA");
+  static_assert(IN != kInF32 || !H16 || AR == 2 || AR == 4, "FP32 A item");
+  static_assert(LDS_BYTES <= 160 * 1024, "LDS");
+};
+
+// Prefetched slice data: FP32 values, or two 16-bit inputs per 32-bit word
+// (element 2w in the low half), widened only when summed.
+template <class C>
+struct LRaw {
+  static constexpr int PW = C::IN == kInF32 ? 1 : 2;   // elements per word
+  typedef typename std::conditional<C::IN == kInF32, float, unsigned int>::type W;
+  W a[C::CH][C::NA][C::AK][C::AR / PW];
+  W b[C::CH][C::NB][C::BKV / PW];
+};
+
+template <class C>
+struct LSum {
+  float a[C::NA][C::AK][C::AR];
+  float b[C::NB][C::BKV];
+};
+
+template <class C>
+struct LGeo {
+  int row0, col0, depth;
+  bool interior;
+  long long aoff[C::NA];   // element offset of A item i at k = 0 within a block
+  long long boff[C::NB];
+  int ar[C::NA], ak[C::NA];   // item row / first k within the tile slice
+  int bj[C::NB], bk[C::NB];   // item column / first k
+};
+
+template <class C>
+struct LSrcs {
+  const typename InT<C::IN>::T* a[C::CH];
+  const typename InT<C::IN>::T* b[C::CH];
+  float u[C::CH];
+  float v[C::CH];
+  int na, nb;
+};
+
+template <class C>
+__device__ __forceinline__ void l_make_srcs(const Job& J, int ta0, int na, int tb0, int nb, LSrcs<C>& S) {
+  typedef typename InT<C::IN>::T T;
+  S.na = na < C::CH ? na : C::CH;
+  S.nb = nb < C::CH ? nb : C::CH;
+#pragma unroll
+  for (int t = 0; t < C::CH; ++t) {
+    if (t < S.na) {
+      const Term tm = term_at(J, ta0 + t);
+      const T* p = static_cast<const T*>(J.A) + (long long)tm.bi * J.ms + (long long)tm.bj * J.ks * J.lda;
+      S.a[t] = unip(p);
+      S.u[t] = (float)unid(tm.coef);
+    } else {
+      S.a[t] = nullptr;
+      S.u[t] = 0.f;
+    }
+    if (t < S.nb) {
+      const Term tm = term_at(J, tb0 + t);
+      const T* p = static_cast<const T*>(J.B) + (long long)tm.bi * J.ks + (long long)tm.bj * J.ns * J.ldb;
+      S.b[t] = unip(p);
+      S.v[t] = (float)unid(tm.coef);
+    } else {
+      S.b[t] = nullptr;
+      S.v[t] = 0.f;
+    }
+  }
+}
+
+// Load N consecutive inputs into raw words (N*size in {8, 16, 32} bytes,
+// aligned to min(16, N*size)).
+template <int IN, int N, class W>
+__device__ __forceinline__ void load_words(const typename InT<IN>::T* p, W* out) {
+  if constexpr (IN == kInF32) {
+    if constexpr (N == 2) {
+      const f32x2 v = *(gcf32x2*)p;
+      out[0] = v.x;
+      out[1] = v.y;
+    } else {
+#pragma unroll
+      for (int q = 0; q < N / 4; ++q) {
+        const f32x4 v = *(gcf32x4*)(p + 4 * q);
+        out[4 * q] = v.x;
+        out[4 * q + 1] = v.y;
+        out[4 * q + 2] = v.z;
+        out[4 * q + 3] = v.w;
+      }
+    }
+  } else {
+    if constexpr (N == 4) {
+      const u32x2 v = *(gcuint2*)p;
+      out[0] = v.x;
+      out[1] = v.y;
+    } else {
+#pragma unroll
+      for (int q = 0; q < N / 8; ++q) {
+        const u32x4 v = *(gcuint4*)(p + 8 * q);
+        out[4 * q] = v.x;
+        out[4 * q + 1] = v.y;
+        out[4 * q + 2] = v.z;
+        out[4 * q + 3] = v.w;
+      }
+    }
+  }
+}
+
+// Element-wise load of one input (zero when !ok) into word slot `e` of `w`.
+template <int IN, class W>
+__device__ __forceinline__ void put_elem(W* w, int e, bool ok, const typename InT<IN>::T* p) {
+  typedef typename InT<IN>::T T;
+  if constexpr (IN == kInF32) {
+    w[e] = ok ? *(const __attribute__((address_space(1))) T*)p : 0.f;
+  } else {
+    const unsigned int v = ok ? (unsigned int)*(const __attribute__((address_space(1))) T*)p : 0u;
+    if (e & 1) w[e >> 1] |= v << 16;
+    else w[e >> 1] = v;
+  }
+}
+
+// Global loads of slice kb (zero outside the leaf block and the depth).
+template <class C>
+__device__ __forceinline__ void l_load_raw(const Job& J, const LSrcs<C>& S, int kb, const LGeo<C>& g, LRaw<C>& r) {
+  typedef typename InT<C::IN>::T T;
+  const bool full = g.interior && kb + C::BK <= g.depth;
+  if (full) {
+    const long long ka = (long long)kb * J.lda;
+#pragma unroll
+    for (int t = 0; t < C::CH; ++t) {
+      if (t < S.na) {
+#pragma unroll
+        for (int i = 0; i < C::NA; ++i)
+#pragma unroll
+          for (int e = 0; e < C::AK; ++e)
+            load_words<C::IN, C::AR>(S.a[t] + ka + g.aoff[i] + (long long)e * J.lda, r.a[t][i][e]);
+      }
+      if (t < S.nb) {
+#pragma unroll
+        for (int i = 0; i < C::NB; ++i) load_words<C::IN, C::BKV>(S.b[t] + kb + g.boff[i], r.b[t][i]);
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int t = 0; t < C::CH; ++t) {
+    if (t < S.na) {
+#pragma unroll
+      for (int i = 0; i < C::NA; ++i)
+#pragma unroll
+        for (int e = 0; e < C::AK; ++e) {
+          const bool kok = kb + g.ak[i] + e < g.depth;
+#pragma unroll
+          for (int q = 0; q < C::AR; ++q) {
+            const bool ok = kok && g.row0 + g.ar[i] + q < J.ms;
+            put_elem<C::IN>(r.a[t][i][e], q, ok, S.a[t] + g.aoff[i] + (long long)(kb + e) * J.lda + q);
+          }
+        }
+    }
+    if (t < S.nb) {
+#pragma unroll
+      for (int i = 0; i < C::NB; ++i) {
+        const bool cok = g.col0 + g.bj[i] < J.ns;
+        const int krem = g.depth - (kb + g.bk[i]);
+#pragma unroll
+        for (int e = 0; e < C::BKV; ++e) put_elem<C::IN>(r.b[t][i], e, cok && e < krem, S.b[t] + kb + g.boff[i] + e);
+      }
+    }
+  }
+}
+
+// Widen raw element e of a word array.
+template <int IN, class W>
+__device__ __forceinline__ float raw_at(const W* w, int e) {
+  if constexpr (IN == kInF32) return w[e];
+  else if constexpr (IN == kInBF16) return (e & 1) ? __uint_as_float(w[e >> 1] & 0xffff0000u) : __uint_as_float(w[e >> 1] << 16);
+  else return f16_bits_to_f((e & 1) ? (w[e >> 1] >> 16) : w[e >> 1]);
+}
+
+template <class C>
+__device__ __forceinline__ void l_accumulate(const LSrcs<C>& S, bool first, const LRaw<C>& r, LSum<C>& s) {
+#pragma unroll
+  for (int t = 0; t < C::CH; ++t) {
+    if (t < S.na) {
+#pragma unroll
+      for (int i = 0; i < C::NA; ++i)
+#pragma unroll
+        for (int e = 0; e < C::AK; ++e)
+#pragma unroll
+          for (int q = 0; q < C::AR; ++q)
+            s.a[i][e][q] = (first && t == 0) ? S.u[t] * raw_at<C::IN>(r.a[t][i][e], q)
+                                             : __builtin_fmaf(S.u[t], raw_at<C::IN>(r.a[t][i][e], q), s.a[i][e][q]);
+    }
+    if (t < S.nb) {
+#pragma unroll
+      for (int i = 0; i < C::NB; ++i)
+#pragma unroll
+        for (int e = 0; e < C::BKV; ++e)
+          s.b[i][e] = (first && t == 0) ? S.v[t] * raw_at<C::IN>(r.b[t][i], e)
+                                        : __builtin_fmaf(S.v[t], raw_at<C::IN>(r.b[t][i], e), s.b[i][e]);
+    }
+  }
+}
+
+template <class C>
+__device__ __forceinline__ void l_finish_sums(const Job& J, const Prod& P, const LSrcs<C>& S0, int kb, const LGeo<C>& g,
+                                              LRaw<C>& r, LSum<C>& s) {
+  l_accumulate<C>(S0, true, r, s);
+  const int nmax = P.na > P.nb ? P.na : P.nb;
+  for (int c = C::CH; c < nmax; c += C::CH) {
+    LSrcs<C> S;
+    l_make_srcs<C>(J, P.a0 + c, P.na - c, P.b0 + c, P.nb - c, S);
+    l_load_raw<C>(J, S, kb, g, r);
+    l_accumulate<C>(S, false, r, s);
+  }
+}
+
+// Round 8 FP32 values to the 16-bit operand type (hi plane) and, for split
+// products, the remainders (lo plane).
+template <int OP>
+__device__ __forceinline__ u32x4 pack8(const float (&x)[8]) {
+  if constexpr (OP == kOpBF16) {
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (__bf16)x[e];
+    return __builtin_bit_cast(u32x4, v);
+  } else {
+    f16x8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (_Float16)x[e];
+    return __builtin_bit_cast(u32x4, v);
+  }
+}
+
+__device__ __forceinline__ void split8(const float (&x)[8], u32x4& hi, u32x4& lo) {
+  bf16x8 h, l;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    h[e] = (__bf16)x[e];
+    l[e] = (__bf16)(x[e] - (float)h[e]);
+  }
+  hi = __builtin_bit_cast(u32x4, h);
+  lo = __builtin_bit_cast(u32x4, l);
+}
+
+// Store the operand sums of one slice into LDS buffer (As, Bs).
+template <class C>
+__device__ __forceinline__ void l_store_slice(char* As, char* Bs, const LGeo<C>& g, const LSum<C>& s) {
+  if constexpr (C::H16) {
+    u16* a = reinterpret_cast<u16*>(As);
+    u16* b = reinterpret_cast<u16*>(Bs);
+#pragma unroll
+    for (int i = 0; i < C::NA; ++i)
+#pragma unroll
+      for (int q = 0; q < C::AR; ++q) {
+        float x[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) x[e] = s.a[i][e][q];
+        const int off = (g.ar[i] + q) * C::SA + g.ak[i];
+        if constexpr (C::SPLIT) {
+          u32x4 hi, lo;
+          split8(x, hi, lo);
+          *reinterpret_cast<u32x4*>(a + off) = hi;
+          *reinterpret_cast<u32x4*>(a + C::A_ELEMS + off) = lo;
+        } else {
+          *reinterpret_cast<u32x4*>(a + off) = pack8<C::OP>(x);
+        }
+      }
+#pragma unroll
+    for (int i = 0; i < C::NB; ++i) {
+      float x[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) x[e] = s.b[i][e];
+      const int off = g.bj[i] * C::SB + g.bk[i];
+      if constexpr (C::SPLIT) {
+        u32x4 hi, lo;
+        split8(x, hi, lo);
+        *reinterpret_cast<u32x4*>(b + off) = hi;
+        *reinterpret_cast<u32x4*>(b + C::B_ELEMS + off) = lo;
+      } else {
+        *reinterpret_cast<u32x4*>(b + off) = pack8<C::OP>(x);
+      }
+    }
+  } else {
+    float* a = reinterpret_cast<float*>(As);
+    float* b = reinterpret_cast<float*>(Bs);
+#pragma unroll
+    for (int i = 0; i < C::NA; ++i) {
+      f32x4 v = {s.a[i][0][0], s.a[i][0][1], s.a[i][0][2], s.a[i][0][3]};
+      *reinterpret_cast<f32x4*>(a + g.ak[i] * C::SA + g.ar[i]) = v;
+    }
+#pragma unroll
+    for (int i = 0; i < C::NB; ++i) {
+      float* p = b + g.bj[i] * C::SB + g.bk[i];
+      f32x2 v0 = {s.b[i][0], s.b[i][1]};
+      f32x2 v1 = {s.b[i][2], s.b[i][3]};
+      *reinterpret_cast<f32x2*>(p) = v0;
+      *reinterpret_cast<f32x2*>(p + 2) = v1;
+    }
+  }
+}
+
+// MFMAs over k-steps [K0, K1) of the slice in LDS buffer (As, Bs).
+template <class C, int K0, int K1>
+__device__ __forceinline__ void l_mfma(const char* As, const char* Bs, int lane, int wm, int wn,
+                                       f32x4 (&acc)[C::FM][C::FN]) {
+  const int l16 = lane & 15, lk = lane >> 4;
+  __builtin_amdgcn_s_setprio(1);
+  if constexpr (C::H16) {
+    const u16* a = reinterpret_cast<const u16*>(As);
+    const u16* b = reinterpret_cast<const u16*>(Bs);
+#pragma unroll
+    for (int kk = K0; kk < K1; ++kk) {
+      const int k0 = kk * 32 + 8 * lk;
+      u32x4 bh[C::FN], bl[C::FN];
+#pragma unroll
+      for (int ni = 0; ni < C::FN; ++ni) {
+        const int off = (wn * C::WTN + ni * 16 + l16) * C::SB + k0;
+        bh[ni] = *reinterpret_cast<const u32x4*>(b + off);
+        if constexpr (C::SPLIT) bl[ni] = *reinterpret_cast<const u32x4*>(b + C::B_ELEMS + off);
+      }
+#pragma unroll
+      for (int mi = 0; mi < C::FM; ++mi) {
+        const int off = (wm * C::WTM + mi * 16 + l16) * C::SA + k0;
+        const u32x4 ah = *reinterpret_cast<const u32x4*>(a + off);
+        if constexpr (C::SPLIT) {
+          const u32x4 al = *reinterpret_cast<const u32x4*>(a + C::A_ELEMS + off);
+#pragma unroll
+          for (int ni = 0; ni < C::FN; ++ni) {
+            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, al),
+                                                                  __builtin_bit_cast(bf16x8, bh[ni]), acc[mi][ni], 0, 0, 0);
+            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah),
+                                                                  __builtin_bit_cast(bf16x8, bl[ni]), acc[mi][ni], 0, 0, 0);
+            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah),
+                                                                  __builtin_bit_cast(bf16x8, bh[ni]), acc[mi][ni], 0, 0, 0);
+          }
+        } else if constexpr (C::OP == kOpBF16) {
+#pragma unroll
+          for (int ni = 0; ni < C::FN; ++ni)
+            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ah),
+                                                                  __builtin_bit_cast(bf16x8, bh[ni]), acc[mi][ni], 0, 0, 0);
+        } else {
+#pragma unroll
+          for (int ni = 0; ni < C::FN; ++ni)
+            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, ah),
+                                                                 __builtin_bit_cast(f16x8, bh[ni]), acc[mi][ni], 0, 0, 0);
+        }
+      }
+    }
+  } else {
+    const float* a = reinterpret_cast<const float*>(As);
+    const float* b = reinterpret_cast<const float*>(Bs);
+#pragma unroll
+    for (int kk = K0; kk < K1; ++kk) {
+      const int k = kk * 4 + lk;
+      float bv[C::FN];
+#pragma unroll
+      for (int ni = 0; ni < C::FN; ++ni) bv[ni] = b[(wn * C::WTN + ni * 16 + l16) * C::SB + k];
+#pragma unroll
+      for (int mi = 0; mi < C::FM; ++mi) {
+        const float av = a[k * C::SA + wm * C::WTM + mi * 16 + l16];
+#pragma unroll
+        for (int ni = 0; ni < C::FN; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[ni], acc[mi][ni], 0, 0, 0);
+      }
+    }
+  }
+  __builtin_amdgcn_s_setprio(0);
+}
+
+template <class C>
+__device__ __forceinline__ void l_sums_phase(const Job& J, const Prod& P, const LSrcs<C>& S, int st, int nsteps,
+                                             const LGeo<C>& g, LRaw<C>& r, LSum<C>& s, char* As, char* Bs) {
+  if (st + 1 < nsteps) {
+    l_finish_sums<C>(J, P, S, (st + 1) * C::BK, g, r, s);
+    l_store_slice<C>(As, Bs, g, s);
+    if (st + 2 < nsteps) l_load_raw<C>(J, S, (st + 2) * C::BK, g, r);
+  }
+}
+
+// Contraction of one unit: double-buffered LDS slices, one barrier per slice;
+// the loads of slice st+2 are in flight while slice st is multiplied.  The two
+// halves of the workgroup's waves run the phases in opposite order, so one
+// half's packing overlaps the other half's MFMAs on the same SIMD.
+template <class C>
+__device__ __forceinline__ void l_contract(const Job& J, const Prod& P, const LGeo<C>& g, char* lds, int lane, int wm,
+                                           int wn, int grp, f32x4 (&acc)[C::FM][C::FN]) {
+  constexpr int KS = C::BK / C::KM;   // MFMA k-steps per slice
+  constexpr int BUF = C::NPL * (C::A_ELEMS + C::B_ELEMS) * C::OPSZ;
+  constexpr int BOFF = C::NPL * C::A_ELEMS * C::OPSZ;
+  const int nsteps = (g.depth + C::BK - 1) / C::BK;
+  if (!(P.na > 0 && P.nb > 0 && nsteps > 0)) return;
+  LSrcs<C> S;
+  l_make_srcs<C>(J, P.a0, P.na, P.b0, P.nb, S);
+  LRaw<C> r;
+  LSum<C> s;
+  l_load_raw<C>(J, S, 0, g, r);
+  l_finish_sums<C>(J, P, S, 0, g, r, s);
+  l_store_slice<C>(lds, lds + BOFF, g, s);
+  if (nsteps > 1) l_load_raw<C>(J, S, C::BK, g, r);
+  __syncthreads();
+  for (int st = 0; st < nsteps; ++st) {
+    char* cur = lds + (st & 1) * BUF;
+    char* nxt = lds + ((st & 1) ^ 1) * BUF;
+    if (grp == 0) {
+      l_mfma<C, 0, KS / 2>(cur, cur + BOFF, lane, wm, wn, acc);
+      l_sums_phase<C>(J, P, S, st, nsteps, g, r, s, nxt, nxt + BOFF);
+      l_mfma<C, KS / 2, KS>(cur, cur + BOFF, lane, wm, wn, acc);
+    } else {
+      l_sums_phase<C>(J, P, S, st, nsteps, g, r, s, nxt, nxt + BOFF);
+      l_mfma<C, 0, KS>(cur, cur + BOFF, lane, wm, wn, acc);
+    }
+    __syncthreads();
+  }
+}
+
+// C_j = C_j + w_j * acc (C_j = w_j * acc for a store job) in FP32: lane l
+// holds rows 4*(l/16)+0..3 of column l%16 of each 16x16 block.
+template <class C>
+__device__ __forceinline__ void l_scatter(const Job& J, const Prod& P, int row0, int col0, int lane, int wm, int wn,
+                                          const f32x4 (&acc)[C::FM][C::FN]) {
+  const int l16 = lane & 15, lk = lane >> 4;
+  const bool store = (J.flags & kJobStore) != 0;
+  const bool cvec = (J.flags & 4) != 0;
+  for (int t = 0; t < P.nc; ++t) {
+    const Term tm = term_at(J, P.c0 + t);
+    const float w = (float)tm.coef;
+    gfloat* cb = (gfloat*)static_cast<float*>(J.C) + (long long)tm.bi * J.ms + (long long)tm.bj * J.ns * J.ldc;
+#pragma unroll
+    for (int mi = 0; mi < C::FM; ++mi) {
+      const int row = row0 + wm * C::WTM + mi * 16 + 4 * lk;
+      if (row >= J.ms) continue;
+#pragma unroll
+      for (int ni = 0; ni < C::FN; ++ni) {
+        const int col = col0 + wn * C::WTN + ni * 16 + l16;
+        if (col >= J.ns) continue;
+        gfloat* c = cb + row + (long long)col * J.ldc;
+        const f32x4 v = acc[mi][ni];
+        if (cvec && row + 4 <= J.ms) {
+          f32x4 o;
+          if (store) {
+            o = f32x4{w * v.x, w * v.y, w * v.z, w * v.w};
+          } else {
+            const f32x4 x = *(gf32x4*)c;
+            o.x = __builtin_fmaf(w, v.x, x.x);
+            o.y = __builtin_fmaf(w, v.y, x.y);
+            o.z = __builtin_fmaf(w, v.z, x.z);
+            o.w = __builtin_fmaf(w, v.w, x.w);
+          }
+          *(gf32x4*)c = o;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (row + e < J.ms) c[e] = store ? w * v[e] : __builtin_fmaf(w, v[e], c[e]);
+        }
+      }
+    }
+  }
+}
+
+template <class C>
+__global__ void __launch_bounds__(C::NT) abc_lowp_kernel(Launch L) {
+  __shared__ __align__(16) char lds[C::LDS_BYTES];
+  __shared__ int s_unit;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % C::WM, wn = wave / C::WM;
+  const int x = (int)(blockIdx.x % (unsigned)L.nq);
+  const int wq = (int)(blockIdx.x / (unsigned)L.nq);
+  const int nw = (int)(gridDim.x / (unsigned)L.nq);
+  if (L.local_handover && tid == 0 && xcc_id() != x) atomicOr(L.ws + kWsError, (int)kErrPlacement);
+
+  int iter = 0;
+  while (true) {
+    int i;
+    if (L.use_ticket) {
+      if (tid == 0) s_unit = atomicAdd(L.ws + kWsTicket * x, 1);
+      __syncthreads();
+      i = __builtin_amdgcn_readfirstlane(s_unit);
+      __syncthreads();
+    } else {
+      i = wq + iter * nw;
+    }
+    ++iter;
+    int j, q, p;
+    if (!decode_unit(L, x, i, &j, &q, &p)) break;
+    j = __builtin_amdgcn_readfirstlane(j);
+    q = __builtin_amdgcn_readfirstlane(q);
+    p = __builtin_amdgcn_readfirstlane(p);
+
+    const Job& J = job_at(L, j);
+    const int tmi = p / J.tiles_n;
+    const int tni = p - tmi * J.tiles_n;
+    LGeo<C> g;
+    g.row0 = tmi * C::BM;
+    g.col0 = tni * C::BN;
+    g.interior = (J.flags & 3) == 3 && g.row0 + C::BM <= J.ms && g.col0 + C::BN <= J.ns;
+#pragma unroll
+    for (int it = 0; it < C::NA; ++it) {
+      const int v = tid + C::NT * it;
+      constexpr int RG = C::BM / C::AR;   // row groups per slice
+      g.ar[it] = (v % RG) * C::AR;
+      g.ak[it] = (v / RG) * C::AK;
+      g.aoff[it] = g.row0 + g.ar[it] + (long long)g.ak[it] * J.lda;
+    }
+#pragma unroll
+    for (int it = 0; it < C::NB; ++it) {
+      const int v = tid + C::NT * it;
+      constexpr int KG = C::BK / C::BKV;   // k groups per column
+      g.bk[it] = (v % KG) * C::BKV;
+      g.bj[it] = v / KG;
+      g.boff[it] = (long long)(g.col0 + g.bj[it]) * J.ldb + g.bk[it];
+    }
+
+    const bool ordered = J.order_base >= 0;
+    int* ctr = L.ws + kWsCounters + (ordered ? J.order_base + p : 0);
+    const int q0 = q * J.gsize;
+    const int q1 = min(J.nprod, q0 + J.gsize);
+    for (int qq = q0; qq < q1; ++qq) {
+      Prod P = prod_at(J, qq);
+      P.a0 = __builtin_amdgcn_readfirstlane(P.a0);
+      P.na = __builtin_amdgcn_readfirstlane(P.na);
+      P.b0 = __builtin_amdgcn_readfirstlane(P.b0);
+      P.nb = __builtin_amdgcn_readfirstlane(P.nb);
+      P.c0 = __builtin_amdgcn_readfirstlane(P.c0);
+      P.nc = __builtin_amdgcn_readfirstlane(P.nc);
+      P.fringe = __builtin_amdgcn_readfirstlane(P.fringe);
+      g.depth = __builtin_amdgcn_readfirstlane(P.fringe ? J.kr : J.ks);
+
+      f32x4 acc[C::FM][C::FN];
+#pragma unroll
+      for (int mi = 0; mi < C::FM; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < C::FN; ++ni) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+      l_contract<C>(J, P, g, lds, lane, wm, wn, wave / (C::WM * C::WN / 2), acc);
+
+      if (ordered && q > 0 && qq == q0) wait_turn(ctr, q, L);
+      l_scatter<C>(J, P, g.row0, g.col0, lane, wm, wn, acc);
+    }
+    if (ordered) pass_turn(ctr, q + 1, L);
+  }
+}
+
+// The configurations built: 256x128 C tiles, 8 waves of 64x64.
+typedef LCfg<kInF32, kOpF32, false, 256, 128, 32, 4, 2, 1> CfgF32;
+typedef LCfg<kInBF16, kOpBF16, false, 256, 128, 64, 4, 2, 1> CfgBF16;
+typedef LCfg<kInF16, kOpF16, false, 256, 128, 64, 4, 2, 1> CfgF16;
+typedef LCfg<kInF32, kOpBF16, true, 256, 128, 32, 4, 2, 1> CfgBF16x3;
+
+}  // namespace fmm

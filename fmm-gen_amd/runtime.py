@@ -22,7 +22,8 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(PKG_DIR, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libfmm_amd.so")
 
-FMM_F64 = 0
+FMM_F64, FMM_F32, FMM_BF16, FMM_F16, FMM_BF16X3 = 0, 1, 2, 3, 4
+DTYPE_NAMES = {FMM_F64: "f64", FMM_F32: "f32", FMM_BF16: "bf16", FMM_F16: "f16", FMM_BF16X3: "bf16x3"}
 PACKS = {"abc": 0, "ab": 1, "naive": 2}
 STATUS = {0: "FMM_OK", -1: "FMM_EINVAL", -2: "FMM_EHIP", -3: "FMM_ENOMEM",
           -4: "FMM_EUNSUPPORTED", -5: "FMM_ESPIN"}
@@ -133,13 +134,35 @@ def _ld(x) -> int:
     return max(int(x.stride(1)), x.shape[0], 1) if x.shape[1] > 1 else max(x.shape[0], 1)
 
 
-def _check_operands(A, B, C):
+def dtype_code(A, B, C, split: bool = False) -> int:
+    """The C-ABI dtype of operands A, B, C: FP64 (C FP64), or FP32 / BF16 /
+    FP16 inputs with an FP32 C; split=True asks for split-BF16 products on
+    FP32 inputs."""
     import torch
+    if A.dtype != B.dtype:
+        raise FmmError(f"A and B must have one dtype (got {A.dtype}, {B.dtype})")
+    if A.dtype == torch.float64:
+        if C.dtype != torch.float64:
+            raise FmmError("C must be float64 for float64 A, B")
+        if split:
+            raise FmmError("split products take float32 inputs")
+        return FMM_F64
+    if C.dtype != torch.float32:
+        raise FmmError(f"C must be float32 for {A.dtype} A, B")
+    code = {torch.float32: FMM_F32, torch.bfloat16: FMM_BF16, torch.float16: FMM_F16}.get(A.dtype)
+    if code is None:
+        raise FmmError(f"unsupported dtype {A.dtype}")
+    if split:
+        if code != FMM_F32:
+            raise FmmError("split products take float32 inputs")
+        return FMM_BF16X3
+    return code
+
+
+def _check_operands(A, B, C):
     for name, X in (("A", A), ("B", B), ("C", C)):
         if not X.is_cuda:
             raise FmmError(f"{name} must be a device tensor")
-        if X.dtype != torch.float64:
-            raise FmmError(f"{name} must be float64 for this entry point")
     m, k = A.shape
     k2, n = B.shape
     if k2 != k or tuple(C.shape) != (m, n):
@@ -179,22 +202,25 @@ class Plan:
             return int(self._lib.fmm_workspace_bytes(self.handle, m, n, k))
         return int(self._lib.fmm_workspace_bytes_pack(self.handle, PACKS[pack], m, n, k))
 
-    def run(self, A, B, C, pack: str = "abc", ws=None, stream=None) -> None:
+    def run(self, A, B, C, pack: str = "abc", ws=None, stream=None, split: bool = False) -> None:
         """C += A @ B on the GPU by the FMM of pack type `pack` ('abc', 'ab'
-        or 'naive'; FP64 device tensors)."""
+        or 'naive').  Device tensors: FP64 A, B, C; or FP32 / BF16 / FP16 A, B
+        with FP32 C (ABC only); split=True: split-BF16 products of FP32 A, B."""
         m, n, k = _check_operands(A, B, C)
+        dt = dtype_code(A, B, C, split)
         wptr, wbytes = (None, 0) if ws is None else (ws.data_ptr(), ws.numel() * ws.element_size())
-        check(self._lib.fmm_run(self.handle, PACKS[pack], FMM_F64, m, n, k, A.data_ptr(), _ld(A),
+        check(self._lib.fmm_run(self.handle, PACKS[pack], dt, m, n, k, A.data_ptr(), _ld(A),
                                 B.data_ptr(), _ld(B), C.data_ptr(), _ld(C), wptr, wbytes,
-                                _stream_ptr(stream)), f"fmm_run({pack})")
+                                _stream_ptr(stream)), f"fmm_run({pack}, {DTYPE_NAMES[dt]})")
 
-    def abc(self, A, B, C, ws=None, stream=None) -> None:
-        """C += A @ B on the GPU by the ABC FMM (FP64, device tensors)."""
+    def abc(self, A, B, C, ws=None, stream=None, split: bool = False) -> None:
+        """C += A @ B on the GPU by the ABC FMM (device tensors; dtypes as run())."""
         m, n, k = _check_operands(A, B, C)
+        dt = dtype_code(A, B, C, split)
         wptr, wbytes = (None, 0) if ws is None else (ws.data_ptr(), ws.numel() * ws.element_size())
-        check(self._lib.fmm_abc(self.handle, FMM_F64, m, n, k, A.data_ptr(), _ld(A), B.data_ptr(),
+        check(self._lib.fmm_abc(self.handle, dt, m, n, k, A.data_ptr(), _ld(A), B.data_ptr(),
                                 _ld(B), C.data_ptr(), _ld(C), wptr, wbytes, _stream_ptr(stream)),
-              "fmm_abc")
+              f"fmm_abc({DTYPE_NAMES[dt]})")
 
     def host_run(self, A: np.ndarray, B: np.ndarray, C: np.ndarray, pack: str = "abc") -> None:
         """Host-pointer drop-in on Fortran-ordered float64 numpy arrays."""
@@ -212,12 +238,14 @@ class Plan:
         self.host_run(A, B, C, "abc")
 
 
-def gemm(A, B, C, stream=None, library=None) -> None:
-    """Plain C += A @ B on the same kernel (identity plan)."""
+def gemm(A, B, C, stream=None, library=None, split: bool = False) -> None:
+    """Plain C += A @ B on the same kernel family (identity plan); dtypes as
+    Plan.run()."""
     L = library or lib()
     m, n, k = _check_operands(A, B, C)
-    check(L.fmm_gemm(FMM_F64, m, n, k, A.data_ptr(), _ld(A), B.data_ptr(), _ld(B), C.data_ptr(),
-                     _ld(C), _stream_ptr(stream)), "fmm_gemm")
+    dt = dtype_code(A, B, C, split)
+    check(L.fmm_gemm(dt, m, n, k, A.data_ptr(), _ld(A), B.data_ptr(), _ld(B), C.data_ptr(),
+                     _ld(C), _stream_ptr(stream)), f"fmm_gemm({DTYPE_NAMES[dt]})")
 
 
 def ws_status(ws, library=None) -> int:

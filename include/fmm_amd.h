@@ -29,8 +29,23 @@ extern "C" {
 #define FMM_EUNSUPPORTED (-4)  /* dtype / variant not built                      */
 #define FMM_ESPIN        (-5)  /* a product-order wait exceeded its time bound   */
 
-/* Element types of A and B (C is FP64 for FMM_F64, FP32 otherwise). */
-#define FMM_F64  0
+/* Element types (dtype argument).  A and B hold the named type; C is FP64
+ * for FMM_F64 and FP32 for every other type.  Operand sums are formed in the
+ * input precision's FP32 (FP64 for FMM_F64) and rounded once to the MFMA
+ * operand type; products accumulate in FP32 (FP64 for FMM_F64).
+ *   FMM_F64     FP64 MFMA (v_mfma_f64_16x16x4f64)          -- the drop-in path
+ *   FMM_F32     FP32 MFMA (v_mfma_f32_16x16x4f32)
+ *   FMM_BF16    BF16 MFMA (v_mfma_f32_16x16x32_bf16)
+ *   FMM_F16     FP16 MFMA (v_mfma_f32_16x16x32_f16)
+ *   FMM_BF16X3  FP32 A, B; each operand sum split into BF16 hi + lo and every
+ *               product formed as hi*hi + hi*lo + lo*hi on BF16 MFMA
+ * The reference has FP64 only (common/include/bl_dgemm.h:52-130); the other
+ * types have no counterpart there.  AB and NAIVE pack types are FP64 only. */
+#define FMM_F64     0
+#define FMM_F32     1
+#define FMM_BF16    2
+#define FMM_F16     3
+#define FMM_BF16X3  4
 
 /* Pack types (the reference's abc / ab / naive generators,
  * meta/abc_gen_mix.py, meta/ab_gen_mix.py, meta/naive_gen_mix.py):
@@ -60,7 +75,8 @@ int fmm_plan_destroy(fmm_plan_t plan);
 int fmm_plan_info(fmm_plan_t plan, int *Mx, int *Kx, int *Nx, int *rank);
 
 /* Device workspace (bytes) fmm_abc needs for this shape: a few ints of
- * product-order counters; 0 when no ordering is needed. */
+ * product-order counters; 0 when no ordering is needed.  The count is that of
+ * FMM_F64 and is enough for every dtype (the others use larger C tiles). */
 size_t fmm_workspace_bytes(fmm_plan_t plan, int m, int n, int k);
 /* The same for any pack type: AB needs M (R [+ Mx*Nx] leaf blocks ms x ns),
  * NAIVE also S (R blocks ms x ks) and T (R blocks ks x ns). */
@@ -91,7 +107,7 @@ int fmm_naive(fmm_plan_t plan, int dtype, int m, int n, int k,
               const void *A, int lda, const void *B, int ldb, void *C, int ldc,
               void *ws, size_t ws_bytes, void *stream);
 
-/* Plain GEMM C += A*B on the same kernel (identity plan).
+/* Plain GEMM C += A*B on the same kernel (identity plan), any dtype above.
  * Replaces: bl_dgemm (common/dgemm/my_dgemm.c:115-210). */
 int fmm_gemm(int dtype, int m, int n, int k, const void *A, int lda,
              const void *B, int ldb, void *C, int ldc, void *stream);

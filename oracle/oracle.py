@@ -124,3 +124,111 @@ def set_threads(nt: int) -> None:
 
 def get_threads() -> int:
     return int(lib().oracle_get_threads())
+
+
+# --------------------------------------------------------------------------
+# Low-precision ABC FMM: numpy restatement of the rounding points of the
+# FP32 / BF16 / FP16 / split-BF16 kernels (no reference counterpart; the
+# reference is FP64 only).  The structure is the reference's ABC algorithm
+# (product order r = 0..R-1, then the k-fringe products, then the fringe
+# GEMMs' contribution; meta/abc_gen_mix.py:448-472 and
+# common/dgemm/bl_dgemm_util.c:316-353), and the rounding points are:
+#   S = fl32(sum_t u_t A_t) summed in table order, then rounded to the
+#   operand type (round to nearest even); likewise T;
+#   P = S_op @ T_op evaluated in float64 (the kernel accumulates in FP32; the
+#   difference is bounded by FP32 accumulation error only);
+#   split-BF16: P = S_hi T_hi + S_hi T_lo + S_lo T_hi with S_hi = bf16(S),
+#   S_lo = bf16(S - S_hi);
+#   C_j = fl32(C_j + w_j * fl32(P)) for every C block of the product.
+
+def round_bf16(x: np.ndarray) -> np.ndarray:
+    """float32 -> nearest bfloat16 (ties to even), returned as float32."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    b = x.view(np.uint32).astype(np.uint64)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return r.view(np.float32).reshape(x.shape)
+
+
+def round_op(x: np.ndarray, dtype: str) -> np.ndarray:
+    x = np.asarray(x, dtype=np.float32)
+    if dtype in ("f32", "bf16x3"):
+        return x
+    if dtype == "bf16":
+        return round_bf16(x)
+    if dtype == "f16":
+        return x.astype(np.float16).astype(np.float32)
+    raise ValueError(dtype)
+
+
+def _lowp_product(S: np.ndarray, T: np.ndarray, dtype: str) -> np.ndarray:
+    if dtype == "bf16x3":
+        sh = round_bf16(S)
+        sl = round_bf16(S - sh)
+        th = round_bf16(T)
+        tl = round_bf16(T - th)
+        f = np.float64
+        return sh.astype(f) @ th.astype(f) + sh.astype(f) @ tl.astype(f) + sl.astype(f) @ th.astype(f)
+    So = round_op(S, dtype).astype(np.float64)
+    To = round_op(T, dtype).astype(np.float64)
+    return So @ To
+
+
+def fmm_lowp(alg, A: np.ndarray, B: np.ndarray, C: np.ndarray, dtype: str) -> np.ndarray:
+    """Returns C + A@B by the low-precision ABC FMM of ``alg`` (float32
+    arrays; A and B already hold values of the input type)."""
+    A = np.asarray(A, dtype=np.float32)
+    B = np.asarray(B, dtype=np.float32)
+    C = np.array(C, dtype=np.float32, copy=True)
+    m, k = A.shape
+    n = B.shape[1]
+    Mx, Kx, Nx = alg.dims
+    md, kd, nd = m - m % (Mx * 8), k - k % Kx, n - n % (Nx * 4)
+    ms, ks, ns = md // Mx, kd // Kx, nd // Nx
+    f32 = np.float32
+
+    def upd(Cv, w, P):
+        Cv[...] = (Cv + f32(w) * P.astype(f32)).astype(f32)
+
+    # fringe GEMMs (one-term sums)
+    if n - nd > 0:
+        upd(C[:, nd:], 1.0, _lowp_product(A, B[:, nd:], dtype))
+    if m - md > 0 and nd > 0:
+        upd(C[md:, :nd], 1.0, _lowp_product(A[md:, :], B[:, :nd], dtype))
+    if ms <= 0 or ns <= 0:
+        return C
+    if ks <= 0:
+        upd(C[:md, :nd], 1.0, _lowp_product(A[:md, :], B[:, :nd], dtype))
+        return C
+    U, V, W = alg.as_arrays()
+
+    def blk(X, rows, cols, I, J):
+        return X[I * rows:(I + 1) * rows, J * cols:(J + 1) * cols]
+
+    def opsum(X, rows, cols, table, which, r):
+        s = None
+        for row in range(table.shape[0]):
+            u = table[row, r]
+            if u == 0:
+                continue
+            I, J = alg.block_of_row(which, row)
+            x = blk(X, rows, cols, I, J)
+            s = (f32(u) * x).astype(f32) if s is None else (s + f32(u) * x).astype(f32)
+        return s
+
+    for r in range(alg.rank):
+        S = opsum(A, ms, ks, U, 0, r)
+        T = opsum(B, ks, ns, V, 1, r)
+        P = _lowp_product(S, T, dtype)
+        for row in range(W.shape[0]):
+            w = W[row, r]
+            if w == 0:
+                continue
+            I, J = alg.block_of_row(2, row)
+            upd(blk(C, ms, ns, I, J), w, P)
+    kr = k - kd
+    if kr > 0:
+        for I in range(Mx):
+            for J in range(Nx):
+                P = _lowp_product(A[I * ms:(I + 1) * ms, kd:], B[kd:, J * ns:(J + 1) * ns], dtype)
+                upd(blk(C, ms, ns, I, J), 1.0, P)
+    return C

@@ -34,3 +34,27 @@ def rel_frob(got, A, B, C0):
     ex = exact(A, B, C0)
     den = float(np.sqrt((ex * ex).sum()))
     return float(np.sqrt(((got.astype(np.longdouble) - ex) ** 2).sum())) / max(den, 1e-300)
+
+
+# Low-precision FMM (FP32 C).  Error metric: relative Frobenius error of the
+# result against the FP64 product of the inputs as stored in the input type.
+# Levels 0 (plain GEMM), 1, 2.  Measured with the numpy restatement of the
+# kernels' rounding points (oracle.fmm_lowp) at 768^3 on U[-1,1) inputs:
+#   f32     2.5e-8 / 1.3e-7 / 3.2e-7
+#   bf16    2.5e-8 / 5.8e-3 / 1.2e-2     (GEMM products are exact in FP32)
+#   f16     2.5e-8 / 7.7e-4 / 1.5e-3
+#   bf16x3  3.8e-6 / 1.1e-5 / 2.2e-5
+# The kernels accumulate in FP32 (the restatement in FP64), which adds up to
+# ~1e-6 at the tested depths; the bounds below leave about 4x margin.
+TOL_LOWP = {
+    "f32": {0: 2e-6, 1: 4e-6, 2: 8e-6},
+    "bf16": {0: 2e-6, 1: 2.5e-2, 2: 5e-2},
+    "f16": {0: 2e-6, 1: 3e-3, 2: 6e-3},
+    "bf16x3": {0: 2e-5, 1: 5e-5, 2: 1e-4},
+}
+# GPU result vs the restatement: FP32 accumulation differences only.
+TOL_LOWP_VS_EMU = 4e-6
+
+
+def tol_lowp(dtype: str, nlevels: int) -> float:
+    return TOL_LOWP[dtype][min(nlevels, 2)]
