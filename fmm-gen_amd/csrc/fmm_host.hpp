@@ -602,13 +602,19 @@ inline bool dtype_geo(int dtype, KGeo* g) {
   }
 }
 
+template <class C>
+inline int launch_lowp(const JobList& JL, int* ws, hipStream_t stream) {
+  return launch_jobs_with(abc_lowp_kernel<C>, C::NT, JL, ws, stream);
+}
+
+// Kernel of a dtype (FP64: by the largest operand-sum term count).
 inline int launch_jobs_dtype(int dtype, const JobList& JL, int max_terms, int* ws, hipStream_t stream) {
   switch (dtype) {
     case kF64: return launch_jobs(JL, max_terms, ws, stream);
-    case kF32: return launch_jobs_with(abc_lowp_kernel<CfgF32>, CfgF32::NT, JL, ws, stream);
-    case kBF16: return launch_jobs_with(abc_lowp_kernel<CfgBF16>, CfgBF16::NT, JL, ws, stream);
-    case kF16: return launch_jobs_with(abc_lowp_kernel<CfgF16>, CfgF16::NT, JL, ws, stream);
-    case kBF16x3: return launch_jobs_with(abc_lowp_kernel<CfgBF16x3>, CfgBF16x3::NT, JL, ws, stream);
+    case kF32: return launch_lowp<CfgF32>(JL, ws, stream);
+    case kBF16: return launch_lowp<CfgBF16>(JL, ws, stream);
+    case kF16: return launch_lowp<CfgF16>(JL, ws, stream);
+    case kBF16x3: return launch_lowp<CfgBF16x3>(JL, ws, stream);
     default: return kEUnsupported;
   }
 }

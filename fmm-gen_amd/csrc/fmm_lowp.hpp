@@ -111,9 +111,15 @@ struct LCfg {
   static constexpr int BKV = H16 ? 8 : 4;
   static constexpr int NA = (BM * BK) / (AR * AK * NT);   // A items per thread
   static constexpr int NB = (BN * BK) / (BKV * NT);       // B items per thread
-  // LDS strides (elements of the operand type)
-  static constexpr int SA = H16 ? BK + 8 : BM + 16;
-  static constexpr int SB = H16 ? BK + 8 : BK + 2;
+  // LDS strides (elements of the operand type).  16-bit rows are unpadded
+  // and their 8-element k groups XOR-swizzled (swz()).
+  static constexpr int SA = H16 ? BK : BM + 16;
+  static constexpr int SB = H16 ? BK : BK + 2;
+  static constexpr int NKG = BK / 8;                  // k groups per 16-bit row
+  // Single-term +-1 operands go to LDS without the FP32 round trip when the
+  // input type is the operand type.
+  static constexpr bool RAWCOPY = !SPLIT && ((IN == kInF32 && OP == kOpF32) || (IN == kInBF16 && OP == kOpBF16) ||
+                                             (IN == kInF16 && OP == kOpF16));
   static constexpr int A_ELEMS = H16 ? BM * SA : BK * SA;
   static constexpr int B_ELEMS = BN * SB;
   static constexpr int OPSZ = H16 ? 2 : 4;
@@ -124,7 +130,19 @@ struct LCfg {
   static_assert(!SPLIT || (IN == kInF32 && OP == kOpBF16), "split products take FP32 inputs on BF16 MFMA");
   static_assert(IN != kInF32 || !H16 || AR == 2 || AR == 4, "FP32 A item");
   static_assert(LDS_BYTES <= 160 * 1024, "LDS");
+  static_assert(!H16 || NKG == 4 || NKG == 8, "16-bit slice depth 32 or 64");
 };
+
+// Physical k group of logical k group `kg` in 16-bit LDS row (or column) r.
+// With 8 groups (128-byte rows) the 8 rows of one 128-byte LDS pass, and the
+// two 128-byte halves of a 256-byte pass, all fall on distinct banks for the
+// MFMA reads (16 consecutive rows, one group) and for the transposing A
+// writes (rows AR apart); likewise with 4 groups (64-byte rows).
+template <class C>
+__device__ __forceinline__ int swz(int r, int kg) {
+  if constexpr (C::NKG == 8) return kg ^ ((r ^ (r >> 3)) & 7);
+  else return kg ^ (((r >> 1) ^ (r >> 3)) & 3);
+}
 
 // Prefetched slice data: FP32 values, or two 16-bit inputs per 32-bit word
 // (element 2w in the low half), widened only when summed.
@@ -194,6 +212,7 @@ __device__ __forceinline__ void l_make_srcs(const Job& J, int ta0, int na, int t
 template <int IN, int N, class W>
 __device__ __forceinline__ void load_words(const typename InT<IN>::T* p, W* out) {
   if constexpr (IN == kInF32) {
+    static_assert(N == 2 || N % 4 == 0, "FP32 vector width");
     if constexpr (N == 2) {
       const f32x2 v = *(gcf32x2*)p;
       out[0] = v.x;
@@ -209,7 +228,10 @@ __device__ __forceinline__ void load_words(const typename InT<IN>::T* p, W* out)
       }
     }
   } else {
-    if constexpr (N == 4) {
+    static_assert(N == 2 || N == 4 || N % 8 == 0, "16-bit vector width");
+    if constexpr (N == 2) {
+      out[0] = *(const __attribute__((address_space(1))) unsigned int*)p;
+    } else if constexpr (N == 4) {
       const u32x2 v = *(gcuint2*)p;
       out[0] = v.x;
       out[1] = v.y;
@@ -298,10 +320,11 @@ __device__ __forceinline__ float raw_at(const W* w, int e) {
 }
 
 template <class C>
-__device__ __forceinline__ void l_accumulate(const LSrcs<C>& S, bool first, const LRaw<C>& r, LSum<C>& s) {
+__device__ __forceinline__ void l_accumulate(const LSrcs<C>& S, bool first, const LRaw<C>& r, LSum<C>& s, bool da,
+                                             bool db) {
 #pragma unroll
   for (int t = 0; t < C::CH; ++t) {
-    if (t < S.na) {
+    if (da && t < S.na) {
 #pragma unroll
       for (int i = 0; i < C::NA; ++i)
 #pragma unroll
@@ -311,7 +334,7 @@ __device__ __forceinline__ void l_accumulate(const LSrcs<C>& S, bool first, cons
             s.a[i][e][q] = (first && t == 0) ? S.u[t] * raw_at<C::IN>(r.a[t][i][e], q)
                                              : __builtin_fmaf(S.u[t], raw_at<C::IN>(r.a[t][i][e], q), s.a[i][e][q]);
     }
-    if (t < S.nb) {
+    if (db && t < S.nb) {
 #pragma unroll
       for (int i = 0; i < C::NB; ++i)
 #pragma unroll
@@ -319,19 +342,6 @@ __device__ __forceinline__ void l_accumulate(const LSrcs<C>& S, bool first, cons
           s.b[i][e] = (first && t == 0) ? S.v[t] * raw_at<C::IN>(r.b[t][i], e)
                                         : __builtin_fmaf(S.v[t], raw_at<C::IN>(r.b[t][i], e), s.b[i][e]);
     }
-  }
-}
-
-template <class C>
-__device__ __forceinline__ void l_finish_sums(const Job& J, const Prod& P, const LSrcs<C>& S0, int kb, const LGeo<C>& g,
-                                              LRaw<C>& r, LSum<C>& s) {
-  l_accumulate<C>(S0, true, r, s);
-  const int nmax = P.na > P.nb ? P.na : P.nb;
-  for (int c = C::CH; c < nmax; c += C::CH) {
-    LSrcs<C> S;
-    l_make_srcs<C>(J, P.a0 + c, P.na - c, P.b0 + c, P.nb - c, S);
-    l_load_raw<C>(J, S, kb, g, r);
-    l_accumulate<C>(S, false, r, s);
   }
 }
 
@@ -363,9 +373,30 @@ __device__ __forceinline__ void split8(const float (&x)[8], u32x4& hi, u32x4& lo
   lo = __builtin_bit_cast(u32x4, l);
 }
 
-// Store the operand sums of one slice into LDS buffer (As, Bs).
+// Lo or hi 16-bit half of each of two words, packed (lo-word half first).
+__device__ __forceinline__ unsigned int pick16(unsigned int w0, unsigned int w1, int hi) {
+  return __builtin_amdgcn_perm(w1, w0, hi ? 0x07060302u : 0x05040100u);
+}
+
+// Operand sums of slice kb into LDS buffer (As, Bs).  `r` holds chunk 0 of
+// the slice (issued earlier); products with more than CH terms load their
+// remaining chunks here.  fa / fb: the A / B operand is one term with
+// coefficient +-1 and goes to LDS straight from the raw words (sign flipped
+// for -1); otherwise sums are formed in FP32 and rounded.
 template <class C>
-__device__ __forceinline__ void l_store_slice(char* As, char* Bs, const LGeo<C>& g, const LSum<C>& s) {
+__device__ __forceinline__ void l_pack(const Job& J, const Prod& P, const LSrcs<C>& S0, int kb, const LGeo<C>& g,
+                                       LRaw<C>& r, char* As, char* Bs, bool fa, bool fb) {
+  LSum<C> s;
+  if (!(fa && fb)) {
+    l_accumulate<C>(S0, true, r, s, !fa, !fb);
+    const int nmax = P.na > P.nb ? P.na : P.nb;
+    for (int c = C::CH; c < nmax; c += C::CH) {
+      LSrcs<C> S;
+      l_make_srcs<C>(J, P.a0 + c, P.na - c, P.b0 + c, P.nb - c, S);
+      l_load_raw<C>(J, S, kb, g, r);
+      l_accumulate<C>(S, false, r, s, !fa, !fb);
+    }
+  }
   if constexpr (C::H16) {
     u16* a = reinterpret_cast<u16*>(As);
     u16* b = reinterpret_cast<u16*>(Bs);
@@ -373,10 +404,23 @@ __device__ __forceinline__ void l_store_slice(char* As, char* Bs, const LGeo<C>&
     for (int i = 0; i < C::NA; ++i)
 #pragma unroll
       for (int q = 0; q < C::AR; ++q) {
+        const int row = g.ar[i] + q;
+        const int off = row * C::SA + swz<C>(row, g.ak[i] >> 3) * 8;
+        if constexpr (C::RAWCOPY) {
+          if (fa) {
+            const unsigned int sg = S0.u[0] < 0.f ? 0x80008000u : 0u;
+            u32x4 v;
+            v.x = pick16(r.a[0][i][0][q >> 1], r.a[0][i][1][q >> 1], q & 1) ^ sg;
+            v.y = pick16(r.a[0][i][2][q >> 1], r.a[0][i][3][q >> 1], q & 1) ^ sg;
+            v.z = pick16(r.a[0][i][4][q >> 1], r.a[0][i][5][q >> 1], q & 1) ^ sg;
+            v.w = pick16(r.a[0][i][6][q >> 1], r.a[0][i][7][q >> 1], q & 1) ^ sg;
+            *reinterpret_cast<u32x4*>(a + off) = v;
+            continue;
+          }
+        }
         float x[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) x[e] = s.a[i][e][q];
-        const int off = (g.ar[i] + q) * C::SA + g.ak[i];
         if constexpr (C::SPLIT) {
           u32x4 hi, lo;
           split8(x, hi, lo);
@@ -388,10 +432,22 @@ __device__ __forceinline__ void l_store_slice(char* As, char* Bs, const LGeo<C>&
       }
 #pragma unroll
     for (int i = 0; i < C::NB; ++i) {
+      const int off = g.bj[i] * C::SB + swz<C>(g.bj[i], g.bk[i] >> 3) * 8;
+      if constexpr (C::RAWCOPY) {
+        if (fb) {
+          const unsigned int sg = S0.v[0] < 0.f ? 0x80008000u : 0u;
+          u32x4 v;
+          v.x = r.b[0][i][0] ^ sg;
+          v.y = r.b[0][i][1] ^ sg;
+          v.z = r.b[0][i][2] ^ sg;
+          v.w = r.b[0][i][3] ^ sg;
+          *reinterpret_cast<u32x4*>(b + off) = v;
+          continue;
+        }
+      }
       float x[8];
 #pragma unroll
       for (int e = 0; e < 8; ++e) x[e] = s.b[i][e];
-      const int off = g.bj[i] * C::SB + g.bk[i];
       if constexpr (C::SPLIT) {
         u32x4 hi, lo;
         split8(x, hi, lo);
@@ -406,14 +462,28 @@ __device__ __forceinline__ void l_store_slice(char* As, char* Bs, const LGeo<C>&
     float* b = reinterpret_cast<float*>(Bs);
 #pragma unroll
     for (int i = 0; i < C::NA; ++i) {
-      f32x4 v = {s.a[i][0][0], s.a[i][0][1], s.a[i][0][2], s.a[i][0][3]};
+      f32x4 v;
+      if (C::RAWCOPY && fa) {
+        const float u = S0.u[0];
+        v = f32x4{u * (float)r.a[0][i][0][0], u * (float)r.a[0][i][0][1], u * (float)r.a[0][i][0][2],
+                  u * (float)r.a[0][i][0][3]};
+      } else {
+        v = f32x4{s.a[i][0][0], s.a[i][0][1], s.a[i][0][2], s.a[i][0][3]};
+      }
       *reinterpret_cast<f32x4*>(a + g.ak[i] * C::SA + g.ar[i]) = v;
     }
 #pragma unroll
     for (int i = 0; i < C::NB; ++i) {
       float* p = b + g.bj[i] * C::SB + g.bk[i];
-      f32x2 v0 = {s.b[i][0], s.b[i][1]};
-      f32x2 v1 = {s.b[i][2], s.b[i][3]};
+      f32x2 v0, v1;
+      if (C::RAWCOPY && fb) {
+        const float v = S0.v[0];
+        v0 = f32x2{v * (float)r.b[0][i][0], v * (float)r.b[0][i][1]};
+        v1 = f32x2{v * (float)r.b[0][i][2], v * (float)r.b[0][i][3]};
+      } else {
+        v0 = f32x2{s.b[i][0], s.b[i][1]};
+        v1 = f32x2{s.b[i][2], s.b[i][3]};
+      }
       *reinterpret_cast<f32x2*>(p) = v0;
       *reinterpret_cast<f32x2*>(p + 2) = v1;
     }
@@ -431,17 +501,19 @@ __device__ __forceinline__ void l_mfma(const char* As, const char* Bs, int lane,
     const u16* b = reinterpret_cast<const u16*>(Bs);
 #pragma unroll
     for (int kk = K0; kk < K1; ++kk) {
-      const int k0 = kk * 32 + 8 * lk;
+      const int kg = kk * 4 + lk;
       u32x4 bh[C::FN], bl[C::FN];
 #pragma unroll
       for (int ni = 0; ni < C::FN; ++ni) {
-        const int off = (wn * C::WTN + ni * 16 + l16) * C::SB + k0;
+        const int col = wn * C::WTN + ni * 16 + l16;
+        const int off = col * C::SB + swz<C>(col, kg) * 8;
         bh[ni] = *reinterpret_cast<const u32x4*>(b + off);
         if constexpr (C::SPLIT) bl[ni] = *reinterpret_cast<const u32x4*>(b + C::B_ELEMS + off);
       }
 #pragma unroll
       for (int mi = 0; mi < C::FM; ++mi) {
-        const int off = (wm * C::WTM + mi * 16 + l16) * C::SA + k0;
+        const int row = wm * C::WTM + mi * 16 + l16;
+        const int off = row * C::SA + swz<C>(row, kg) * 8;
         const u32x4 ah = *reinterpret_cast<const u32x4*>(a + off);
         if constexpr (C::SPLIT) {
           const u32x4 al = *reinterpret_cast<const u32x4*>(a + C::A_ELEMS + off);
@@ -490,10 +562,9 @@ __device__ __forceinline__ void l_mfma(const char* As, const char* Bs, int lane,
 
 template <class C>
 __device__ __forceinline__ void l_sums_phase(const Job& J, const Prod& P, const LSrcs<C>& S, int st, int nsteps,
-                                             const LGeo<C>& g, LRaw<C>& r, LSum<C>& s, char* As, char* Bs) {
+                                             const LGeo<C>& g, LRaw<C>& r, char* As, char* Bs, bool fa, bool fb) {
   if (st + 1 < nsteps) {
-    l_finish_sums<C>(J, P, S, (st + 1) * C::BK, g, r, s);
-    l_store_slice<C>(As, Bs, g, s);
+    l_pack<C>(J, P, S, (st + 1) * C::BK, g, r, As, Bs, fa, fb);
     if (st + 2 < nsteps) l_load_raw<C>(J, S, (st + 2) * C::BK, g, r);
   }
 }
@@ -512,11 +583,11 @@ __device__ __forceinline__ void l_contract(const Job& J, const Prod& P, const LG
   if (!(P.na > 0 && P.nb > 0 && nsteps > 0)) return;
   LSrcs<C> S;
   l_make_srcs<C>(J, P.a0, P.na, P.b0, P.nb, S);
+  const bool fa = C::RAWCOPY && P.na == 1 && __builtin_fabsf(S.u[0]) == 1.f;
+  const bool fb = C::RAWCOPY && P.nb == 1 && __builtin_fabsf(S.v[0]) == 1.f;
   LRaw<C> r;
-  LSum<C> s;
   l_load_raw<C>(J, S, 0, g, r);
-  l_finish_sums<C>(J, P, S, 0, g, r, s);
-  l_store_slice<C>(lds, lds + BOFF, g, s);
+  l_pack<C>(J, P, S, 0, g, r, lds, lds + BOFF, fa, fb);
   if (nsteps > 1) l_load_raw<C>(J, S, C::BK, g, r);
   __syncthreads();
   for (int st = 0; st < nsteps; ++st) {
@@ -524,10 +595,10 @@ __device__ __forceinline__ void l_contract(const Job& J, const Prod& P, const LG
     char* nxt = lds + ((st & 1) ^ 1) * BUF;
     if (grp == 0) {
       l_mfma<C, 0, KS / 2>(cur, cur + BOFF, lane, wm, wn, acc);
-      l_sums_phase<C>(J, P, S, st, nsteps, g, r, s, nxt, nxt + BOFF);
+      l_sums_phase<C>(J, P, S, st, nsteps, g, r, nxt, nxt + BOFF, fa, fb);
       l_mfma<C, KS / 2, KS>(cur, cur + BOFF, lane, wm, wn, acc);
     } else {
-      l_sums_phase<C>(J, P, S, st, nsteps, g, r, s, nxt, nxt + BOFF);
+      l_sums_phase<C>(J, P, S, st, nsteps, g, r, nxt, nxt + BOFF, fa, fb);
       l_mfma<C, 0, KS>(cur, cur + BOFF, lane, wm, wn, acc);
     }
     __syncthreads();
@@ -662,10 +733,13 @@ __global__ void __launch_bounds__(C::NT) abc_lowp_kernel(Launch L) {
   }
 }
 
-// The configurations built: 256x128 C tiles, 8 waves of 64x64.
-typedef LCfg<kInF32, kOpF32, false, 256, 128, 32, 4, 2, 1> CfgF32;
-typedef LCfg<kInBF16, kOpBF16, false, 256, 128, 64, 4, 2, 1> CfgBF16;
-typedef LCfg<kInF16, kOpF16, false, 256, 128, 64, 4, 2, 1> CfgF16;
+// The configurations built: 256x128 C tiles, 8 waves of 64x64, the deepest
+// slice and the largest prefetched term chunk that fit the 256 VGPRs of two
+// waves per SIMD without spilling (operand sums with more terms load their
+// remaining chunks synchronously).
+typedef LCfg<kInF32, kOpF32, false, 256, 128, 16, 4, 2, 2> CfgF32;
+typedef LCfg<kInBF16, kOpBF16, false, 256, 128, 32, 4, 2, 2> CfgBF16;
+typedef LCfg<kInF16, kOpF16, false, 256, 128, 32, 4, 2, 2> CfgF16;
 typedef LCfg<kInF32, kOpBF16, true, 256, 128, 32, 4, 2, 1> CfgBF16x3;
 
 }  // namespace fmm
