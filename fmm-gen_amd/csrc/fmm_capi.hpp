@@ -5,11 +5,16 @@
 #pragma once
 #include <cstdlib>
 
+#include "fmm_batch.hpp"
 #include "fmm_host.hpp"
 #include "fmm_amd.h"
 
 struct fmm_plan_s {
   fmm::Plan p;
+};
+
+struct fmm_batch_s {
+  fmm::Batch b;
 };
 
 namespace fmm {
@@ -280,6 +285,59 @@ void bl_dgemm_ref(int m, int n, int k, double* XA, int lda, double* XB, int ldb,
       for (int p = 0; p < k; ++p) c += XA[(size_t)p * lda + i] * XB[(size_t)j * ldb + p];
       XC[(size_t)j * ldc + i] = c;
     }
+}
+
+int fmm_batch_create(fmm_batch_t* out, int dtype, int nprob, const fmm_problem_t* probs, int nplans,
+                     const fmm_plan_t* plans) {
+  if (!out) return fmm::kEInval;
+  *out = nullptr;
+  if (nprob < 0 || (nprob > 0 && !probs) || nplans < 0 || (nplans > 0 && !plans)) return fmm::kEInval;
+  std::vector<fmm::BatchSpec> specs((size_t)nprob);
+  for (int i = 0; i < nprob; ++i) {
+    const fmm_problem_t& p = probs[i];
+    if (p.plan < -1 || p.plan >= nplans || (p.plan >= 0 && !plans[p.plan])) return fmm::kEInval;
+    if (p.tile != 0 && p.tile != 1) return fmm::kEInval;
+    specs[i] = fmm::BatchSpec{p.m, p.n, p.k, p.A, p.lda, p.B, p.ldb, p.C, p.ldc,
+                              p.plan >= 0 ? &plans[p.plan]->p : nullptr, p.tile};
+  }
+  fmm_batch_s* b = new (std::nothrow) fmm_batch_s();
+  if (!b) return fmm::kENoMem;
+  int rc = fmm::batch_build(&b->b, dtype, nprob, specs.data());
+  if (rc != fmm::kOK) {
+    b->b.release();
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return fmm::kOK;
+}
+
+int fmm_batch_run(fmm_batch_t batch, void* stream) {
+  if (!batch) return fmm::kEInval;
+  return fmm::batch_run(batch->b, static_cast<hipStream_t>(stream));
+}
+
+int fmm_batch_info(fmm_batch_t batch, int* jobs, int* units) {
+  if (!batch) return fmm::kEInval;
+  if (jobs) *jobs = (int)batch->b.JL.jobs.size();
+  if (units) *units = batch->b.JL.total_units;
+  return fmm::kOK;
+}
+
+int fmm_batch_status(fmm_batch_t batch, int* flags) {
+  if (!batch) return fmm::kEInval;
+  return fmm_ws_status(batch->b.dws, flags);
+}
+
+int fmm_batch_destroy(fmm_batch_t batch) {
+  if (!batch) return fmm::kEInval;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  if (batch->b.dev >= 0) (void)hipSetDevice(batch->b.dev);
+  batch->b.release();
+  (void)hipSetDevice(cur);
+  delete batch;
+  return fmm::kOK;
 }
 
 void bl_finalize(void) {

@@ -80,6 +80,9 @@ struct Job {
 
 constexpr int kInlineJobs = 4;
 constexpr int kJobStore = 8;
+// Low-precision kernels: run this job with the kernel's second C-tile shape
+// (128 x 256 instead of 256 x 128).
+constexpr int kJobTileT = 16;
 
 struct Launch {
   int njobs;

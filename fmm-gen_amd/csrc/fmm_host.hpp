@@ -591,30 +591,47 @@ constexpr KGeo lgeo() {
   return KGeo{C::BM, C::BN, C::IN == kInF32 ? 4 : 2, 4, C::AR * (C::IN == kInF32 ? 4 : 2)};
 }
 
-inline bool dtype_geo(int dtype, KGeo* g) {
+// Geometry of a dtype's kernel for C-tile shape `tile` (0: the default, 1:
+// the transposed 128x256 shape; types without it fall back to the default).
+// *flag receives the job flag that selects the shape.
+inline bool dtype_geo(int dtype, KGeo* g, int tile = 0, int* flag = nullptr) {
+  int f = 0;
   switch (dtype) {
-    case kF64: *g = kGeoF64; return true;
-    case kF32: *g = lgeo<CfgF32>(); return true;
-    case kBF16: *g = lgeo<CfgBF16>(); return true;
-    case kF16: *g = lgeo<CfgF16>(); return true;
-    case kBF16x3: *g = lgeo<CfgBF16x3>(); return true;
+    case kF64: *g = kGeoF64; break;
+    case kF32: *g = tile ? lgeo<CfgF32T>() : lgeo<CfgF32>(); f = tile ? kJobTileT : 0; break;
+    case kBF16: *g = tile ? lgeo<CfgBF16T>() : lgeo<CfgBF16>(); f = tile ? kJobTileT : 0; break;
+    case kF16: *g = tile ? lgeo<CfgF16T>() : lgeo<CfgF16>(); f = tile ? kJobTileT : 0; break;
+    case kBF16x3: *g = lgeo<CfgBF16x3>(); break;
     default: return false;
   }
+  if (flag) *flag = f;
+  return true;
 }
 
-template <class C>
+template <class C1, class C2>
 inline int launch_lowp(const JobList& JL, int* ws, hipStream_t stream) {
-  return launch_jobs_with(abc_lowp_kernel<C>, C::NT, JL, ws, stream);
+  return launch_jobs_with(abc_lowp_kernel<C1, C2>, C1::NT, JL, ws, stream);
 }
 
-// Kernel of a dtype (FP64: by the largest operand-sum term count).
+// Kernel of a dtype (FP64: by the largest operand-sum term count).  A launch
+// with jobs of both C-tile shapes runs the two-shape kernel; otherwise the
+// one-shape kernel of its shape (fewer live registers).
+template <class C1, class C2>
+inline int launch_lowp_shapes(const JobList& JL, int* ws, hipStream_t stream) {
+  bool t = false, d = false;
+  for (const Job& j : JL.jobs) ((j.flags & kJobTileT) ? t : d) = true;
+  if (t && d) return launch_lowp<C1, C2>(JL, ws, stream);
+  if (t) return launch_lowp<C2, C2>(JL, ws, stream);
+  return launch_lowp<C1, C1>(JL, ws, stream);
+}
+
 inline int launch_jobs_dtype(int dtype, const JobList& JL, int max_terms, int* ws, hipStream_t stream) {
   switch (dtype) {
     case kF64: return launch_jobs(JL, max_terms, ws, stream);
-    case kF32: return launch_lowp<CfgF32>(JL, ws, stream);
-    case kBF16: return launch_lowp<CfgBF16>(JL, ws, stream);
-    case kF16: return launch_lowp<CfgF16>(JL, ws, stream);
-    case kBF16x3: return launch_lowp<CfgBF16x3>(JL, ws, stream);
+    case kF32: return launch_lowp_shapes<CfgF32, CfgF32T>(JL, ws, stream);
+    case kBF16: return launch_lowp_shapes<CfgBF16, CfgBF16T>(JL, ws, stream);
+    case kF16: return launch_lowp_shapes<CfgF16, CfgF16T>(JL, ws, stream);
+    case kBF16x3: return launch_lowp<CfgBF16x3, CfgBF16x3>(JL, ws, stream);
     default: return kEUnsupported;
   }
 }

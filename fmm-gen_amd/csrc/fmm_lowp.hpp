@@ -107,7 +107,7 @@ struct LCfg {
   static constexpr int NPL = SPLIT ? 2 : 1;           // operand planes (hi, lo)
   // A items: AR rows x AK k per item.  B items: BKV consecutive k of one column.
   static constexpr int AR = H16 ? ((BM * BK) / (8 * NT) >= 4 ? 4 : 2) : 4;
-  static constexpr int AK = H16 ? 8 : 1;
+  static constexpr int AK = H16 ? ((BM * BK) / (AR * NT) >= 8 ? 8 : 4) : 1;
   static constexpr int BKV = H16 ? 8 : 4;
   static constexpr int NA = (BM * BK) / (AR * AK * NT);   // A items per thread
   static constexpr int NB = (BN * BK) / (BKV * NT);       // B items per thread
@@ -345,32 +345,40 @@ __device__ __forceinline__ void l_accumulate(const LSrcs<C>& S, bool first, cons
   }
 }
 
-// Round 8 FP32 values to the 16-bit operand type (hi plane) and, for split
-// products, the remainders (lo plane).
+// Round N FP32 values (N = 4 or 8) to the 16-bit operand type, two per word
+// (hi plane) and, for split products, the remainders (lo plane).
 template <int OP>
-__device__ __forceinline__ u32x4 pack8(const float (&x)[8]) {
+__device__ __forceinline__ unsigned int pack2(float x0, float x1) {
   if constexpr (OP == kOpBF16) {
-    bf16x8 v;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = (__bf16)x[e];
-    return __builtin_bit_cast(u32x4, v);
+    const __bf16 a = (__bf16)x0, b = (__bf16)x1;
+    return (unsigned int)__builtin_bit_cast(u16, a) | ((unsigned int)__builtin_bit_cast(u16, b) << 16);
   } else {
-    f16x8 v;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = (_Float16)x[e];
-    return __builtin_bit_cast(u32x4, v);
+    const _Float16 a = (_Float16)x0, b = (_Float16)x1;
+    return (unsigned int)__builtin_bit_cast(u16, a) | ((unsigned int)__builtin_bit_cast(u16, b) << 16);
   }
 }
 
-__device__ __forceinline__ void split8(const float (&x)[8], u32x4& hi, u32x4& lo) {
-  bf16x8 h, l;
+template <int OP, int N>
+__device__ __forceinline__ void packN(const float (&x)[N], unsigned int (&w)[N / 2]) {
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    h[e] = (__bf16)x[e];
-    l[e] = (__bf16)(x[e] - (float)h[e]);
+  for (int e = 0; e < N / 2; ++e) w[e] = pack2<OP>(x[2 * e], x[2 * e + 1]);
+}
+
+template <int N>
+__device__ __forceinline__ void splitN(const float (&x)[N], unsigned int (&hi)[N / 2], unsigned int (&lo)[N / 2]) {
+#pragma unroll
+  for (int e = 0; e < N / 2; ++e) {
+    const __bf16 h0 = (__bf16)x[2 * e], h1 = (__bf16)x[2 * e + 1];
+    hi[e] = (unsigned int)__builtin_bit_cast(u16, h0) | ((unsigned int)__builtin_bit_cast(u16, h1) << 16);
+    lo[e] = pack2<kOpBF16>(x[2 * e] - (float)h0, x[2 * e + 1] - (float)h1);
   }
-  hi = __builtin_bit_cast(u32x4, h);
-  lo = __builtin_bit_cast(u32x4, l);
+}
+
+// Store N/2 words (N = 4: 8 bytes, N = 8: 16 bytes) to LDS.
+template <int N>
+__device__ __forceinline__ void st_words(u16* p, const unsigned int (&w)[N / 2]) {
+  if constexpr (N == 8) *reinterpret_cast<u32x4*>(p) = u32x4{w[0], w[1], w[2], w[3]};
+  else *reinterpret_cast<u32x2*>(p) = u32x2{w[0], w[1]};
 }
 
 // Lo or hi 16-bit half of each of two words, packed (lo-word half first).
@@ -404,30 +412,31 @@ __device__ __forceinline__ void l_pack(const Job& J, const Prod& P, const LSrcs<
     for (int i = 0; i < C::NA; ++i)
 #pragma unroll
       for (int q = 0; q < C::AR; ++q) {
+        constexpr int AK = C::AK;
         const int row = g.ar[i] + q;
-        const int off = row * C::SA + swz<C>(row, g.ak[i] >> 3) * 8;
+        const int off = row * C::SA + swz<C>(row, g.ak[i] >> 3) * 8 + (g.ak[i] & 7);
         if constexpr (C::RAWCOPY) {
           if (fa) {
             const unsigned int sg = S0.u[0] < 0.f ? 0x80008000u : 0u;
-            u32x4 v;
-            v.x = pick16(r.a[0][i][0][q >> 1], r.a[0][i][1][q >> 1], q & 1) ^ sg;
-            v.y = pick16(r.a[0][i][2][q >> 1], r.a[0][i][3][q >> 1], q & 1) ^ sg;
-            v.z = pick16(r.a[0][i][4][q >> 1], r.a[0][i][5][q >> 1], q & 1) ^ sg;
-            v.w = pick16(r.a[0][i][6][q >> 1], r.a[0][i][7][q >> 1], q & 1) ^ sg;
-            *reinterpret_cast<u32x4*>(a + off) = v;
+            unsigned int w[AK / 2];
+#pragma unroll
+            for (int e = 0; e < AK / 2; ++e) w[e] = pick16(r.a[0][i][2 * e][q >> 1], r.a[0][i][2 * e + 1][q >> 1], q & 1) ^ sg;
+            st_words<AK>(a + off, w);
             continue;
           }
         }
-        float x[8];
+        float x[AK];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) x[e] = s.a[i][e][q];
+        for (int e = 0; e < AK; ++e) x[e] = s.a[i][e][q];
         if constexpr (C::SPLIT) {
-          u32x4 hi, lo;
-          split8(x, hi, lo);
-          *reinterpret_cast<u32x4*>(a + off) = hi;
-          *reinterpret_cast<u32x4*>(a + C::A_ELEMS + off) = lo;
+          unsigned int hi[AK / 2], lo[AK / 2];
+          splitN<AK>(x, hi, lo);
+          st_words<AK>(a + off, hi);
+          st_words<AK>(a + C::A_ELEMS + off, lo);
         } else {
-          *reinterpret_cast<u32x4*>(a + off) = pack8<C::OP>(x);
+          unsigned int w[AK / 2];
+          packN<C::OP, AK>(x, w);
+          st_words<AK>(a + off, w);
         }
       }
 #pragma unroll
@@ -449,12 +458,14 @@ __device__ __forceinline__ void l_pack(const Job& J, const Prod& P, const LSrcs<
 #pragma unroll
       for (int e = 0; e < 8; ++e) x[e] = s.b[i][e];
       if constexpr (C::SPLIT) {
-        u32x4 hi, lo;
-        split8(x, hi, lo);
-        *reinterpret_cast<u32x4*>(b + off) = hi;
-        *reinterpret_cast<u32x4*>(b + C::B_ELEMS + off) = lo;
+        unsigned int hi[4], lo[4];
+        splitN<8>(x, hi, lo);
+        st_words<8>(b + off, hi);
+        st_words<8>(b + C::B_ELEMS + off, lo);
       } else {
-        *reinterpret_cast<u32x4*>(b + off) = pack8<C::OP>(x);
+        unsigned int w[4];
+        packN<C::OP, 8>(x, w);
+        st_words<8>(b + off, w);
       }
     }
   } else {
@@ -649,13 +660,72 @@ __device__ __forceinline__ void l_scatter(const Job& J, const Prod& P, int row0,
   }
 }
 
+// One unit (product group q of tile position p of job J) with configuration C.
 template <class C>
-__global__ void __launch_bounds__(C::NT) abc_lowp_kernel(Launch L) {
-  __shared__ __align__(16) char lds[C::LDS_BYTES];
-  __shared__ int s_unit;
-  const int tid = threadIdx.x;
+__device__ __forceinline__ void l_unit(const Launch& L, const Job& J, int q, int p, char* lds, int tid) {
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % C::WM, wn = wave / C::WM;
+  const int tmi = p / J.tiles_n;
+  const int tni = p - tmi * J.tiles_n;
+  LGeo<C> g;
+  g.row0 = tmi * C::BM;
+  g.col0 = tni * C::BN;
+  g.interior = (J.flags & 3) == 3 && g.row0 + C::BM <= J.ms && g.col0 + C::BN <= J.ns;
+#pragma unroll
+  for (int it = 0; it < C::NA; ++it) {
+    const int v = tid + C::NT * it;
+    constexpr int RG = C::BM / C::AR;   // row groups per slice
+    g.ar[it] = (v % RG) * C::AR;
+    g.ak[it] = (v / RG) * C::AK;
+    g.aoff[it] = g.row0 + g.ar[it] + (long long)g.ak[it] * J.lda;
+  }
+#pragma unroll
+  for (int it = 0; it < C::NB; ++it) {
+    const int v = tid + C::NT * it;
+    constexpr int KG = C::BK / C::BKV;   // k groups per column
+    g.bk[it] = (v % KG) * C::BKV;
+    g.bj[it] = v / KG;
+    g.boff[it] = (long long)(g.col0 + g.bj[it]) * J.ldb + g.bk[it];
+  }
+
+  const bool ordered = J.order_base >= 0;
+  int* ctr = L.ws + kWsCounters + (ordered ? J.order_base + p : 0);
+  const int q0 = q * J.gsize;
+  const int q1 = min(J.nprod, q0 + J.gsize);
+  for (int qq = q0; qq < q1; ++qq) {
+    Prod P = prod_at(J, qq);
+    P.a0 = __builtin_amdgcn_readfirstlane(P.a0);
+    P.na = __builtin_amdgcn_readfirstlane(P.na);
+    P.b0 = __builtin_amdgcn_readfirstlane(P.b0);
+    P.nb = __builtin_amdgcn_readfirstlane(P.nb);
+    P.c0 = __builtin_amdgcn_readfirstlane(P.c0);
+    P.nc = __builtin_amdgcn_readfirstlane(P.nc);
+    P.fringe = __builtin_amdgcn_readfirstlane(P.fringe);
+    g.depth = __builtin_amdgcn_readfirstlane(P.fringe ? J.kr : J.ks);
+
+    f32x4 acc[C::FM][C::FN];
+#pragma unroll
+    for (int mi = 0; mi < C::FM; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < C::FN; ++ni) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    l_contract<C>(J, P, g, lds, lane, wm, wn, wave / (C::WM * C::WN / 2), acc);
+
+    if (ordered && q > 0 && qq == q0) wait_turn(ctr, q, L);
+    l_scatter<C>(J, P, g.row0, g.col0, lane, wm, wn, acc);
+  }
+  if (ordered) pass_turn(ctr, q + 1, L);
+}
+
+// The persistent kernel: C1 runs jobs with the default tile shape, C2 the
+// jobs flagged kJobTileT (both: same element types and thread count).
+template <class C1, class C2>
+__global__ void __launch_bounds__(C1::NT) abc_lowp_kernel(Launch L) {
+  static_assert(C1::NT == C2::NT && C1::IN == C2::IN && C1::OP == C2::OP && C1::SPLIT == C2::SPLIT, "kernel pair");
+  constexpr int LDSB = C1::LDS_BYTES > C2::LDS_BYTES ? C1::LDS_BYTES : C2::LDS_BYTES;
+  __shared__ __align__(16) char lds[LDSB];
+  __shared__ int s_unit;
+  const int tid = threadIdx.x;
   const int x = (int)(blockIdx.x % (unsigned)L.nq);
   const int wq = (int)(blockIdx.x / (unsigned)L.nq);
   const int nw = (int)(gridDim.x / (unsigned)L.nq);
@@ -678,68 +748,25 @@ __global__ void __launch_bounds__(C::NT) abc_lowp_kernel(Launch L) {
     j = __builtin_amdgcn_readfirstlane(j);
     q = __builtin_amdgcn_readfirstlane(q);
     p = __builtin_amdgcn_readfirstlane(p);
-
     const Job& J = job_at(L, j);
-    const int tmi = p / J.tiles_n;
-    const int tni = p - tmi * J.tiles_n;
-    LGeo<C> g;
-    g.row0 = tmi * C::BM;
-    g.col0 = tni * C::BN;
-    g.interior = (J.flags & 3) == 3 && g.row0 + C::BM <= J.ms && g.col0 + C::BN <= J.ns;
-#pragma unroll
-    for (int it = 0; it < C::NA; ++it) {
-      const int v = tid + C::NT * it;
-      constexpr int RG = C::BM / C::AR;   // row groups per slice
-      g.ar[it] = (v % RG) * C::AR;
-      g.ak[it] = (v / RG) * C::AK;
-      g.aoff[it] = g.row0 + g.ar[it] + (long long)g.ak[it] * J.lda;
-    }
-#pragma unroll
-    for (int it = 0; it < C::NB; ++it) {
-      const int v = tid + C::NT * it;
-      constexpr int KG = C::BK / C::BKV;   // k groups per column
-      g.bk[it] = (v % KG) * C::BKV;
-      g.bj[it] = v / KG;
-      g.boff[it] = (long long)(g.col0 + g.bj[it]) * J.ldb + g.bk[it];
-    }
-
-    const bool ordered = J.order_base >= 0;
-    int* ctr = L.ws + kWsCounters + (ordered ? J.order_base + p : 0);
-    const int q0 = q * J.gsize;
-    const int q1 = min(J.nprod, q0 + J.gsize);
-    for (int qq = q0; qq < q1; ++qq) {
-      Prod P = prod_at(J, qq);
-      P.a0 = __builtin_amdgcn_readfirstlane(P.a0);
-      P.na = __builtin_amdgcn_readfirstlane(P.na);
-      P.b0 = __builtin_amdgcn_readfirstlane(P.b0);
-      P.nb = __builtin_amdgcn_readfirstlane(P.nb);
-      P.c0 = __builtin_amdgcn_readfirstlane(P.c0);
-      P.nc = __builtin_amdgcn_readfirstlane(P.nc);
-      P.fringe = __builtin_amdgcn_readfirstlane(P.fringe);
-      g.depth = __builtin_amdgcn_readfirstlane(P.fringe ? J.kr : J.ks);
-
-      f32x4 acc[C::FM][C::FN];
-#pragma unroll
-      for (int mi = 0; mi < C::FM; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < C::FN; ++ni) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-      l_contract<C>(J, P, g, lds, lane, wm, wn, wave / (C::WM * C::WN / 2), acc);
-
-      if (ordered && q > 0 && qq == q0) wait_turn(ctr, q, L);
-      l_scatter<C>(J, P, g.row0, g.col0, lane, wm, wn, acc);
-    }
-    if (ordered) pass_turn(ctr, q + 1, L);
+    const int tt = __builtin_amdgcn_readfirstlane(J.flags & kJobTileT);
+    if (tt && !__is_same(C1, C2)) l_unit<C2>(L, J, q, p, lds, tid);
+    else l_unit<C1>(L, J, q, p, lds, tid);
   }
 }
 
-// The configurations built: 256x128 C tiles, 8 waves of 64x64, the deepest
-// slice and the largest prefetched term chunk that fit the 256 VGPRs of two
-// waves per SIMD without spilling (operand sums with more terms load their
-// remaining chunks synchronously).
+// The configurations built: 8 waves of 64x64 per workgroup, C tiles of
+// 256x128 (default) or 128x256 (jobs flagged kJobTileT), each with the
+// deepest slice and the largest prefetched term chunk that fit the 256 VGPRs
+// of two waves per SIMD without spilling (operand sums with more terms load
+// their remaining chunks synchronously).  Split-BF16 has the 256x128 shape
+// only (its hi and lo planes leave no LDS for the deeper 128x256 slices).
 typedef LCfg<kInF32, kOpF32, false, 256, 128, 16, 4, 2, 2> CfgF32;
+typedef LCfg<kInF32, kOpF32, false, 128, 256, 16, 2, 4, 2> CfgF32T;
 typedef LCfg<kInBF16, kOpBF16, false, 256, 128, 32, 4, 2, 2> CfgBF16;
+typedef LCfg<kInBF16, kOpBF16, false, 128, 256, 32, 2, 4, 1> CfgBF16T;
 typedef LCfg<kInF16, kOpF16, false, 256, 128, 32, 4, 2, 2> CfgF16;
+typedef LCfg<kInF16, kOpF16, false, 128, 256, 32, 2, 4, 1> CfgF16T;
 typedef LCfg<kInF32, kOpBF16, true, 256, 128, 32, 4, 2, 1> CfgBF16x3;
 
 }  // namespace fmm

@@ -253,3 +253,92 @@ def ws_status(ws, library=None) -> int:
     flags = ct.c_int(0)
     L.fmm_ws_status(ws.data_ptr(), ct.byref(flags))
     return int(flags.value)
+
+
+class _Problem(ct.Structure):
+    _fields_ = [("m", ct.c_int), ("n", ct.c_int), ("k", ct.c_int),
+                ("A", ct.c_void_p), ("lda", ct.c_int),
+                ("B", ct.c_void_p), ("ldb", ct.c_int),
+                ("C", ct.c_void_p), ("ldc", ct.c_int),
+                ("plan", ct.c_int), ("tile", ct.c_int)]
+
+
+def _declare_batch(L):
+    if getattr(L, "_batch_declared", False):
+        return
+    vp, i = ct.c_void_p, ct.c_int
+    L.fmm_batch_create.argtypes = [ct.POINTER(vp), i, i, ct.POINTER(_Problem), i, ct.POINTER(vp)]
+    L.fmm_batch_create.restype = i
+    L.fmm_batch_run.argtypes = [vp, vp]
+    L.fmm_batch_run.restype = i
+    L.fmm_batch_info.argtypes = [vp, ct.POINTER(i), ct.POINTER(i)]
+    L.fmm_batch_info.restype = i
+    L.fmm_batch_status.argtypes = [vp, ct.POINTER(i)]
+    L.fmm_batch_status.restype = i
+    L.fmm_batch_destroy.argtypes = [vp]
+    L.fmm_batch_destroy.restype = i
+    L._batch_declared = True
+
+
+class Batch:
+    """Many independent C += A @ B problems run by ONE persistent launch.
+
+    ``problems``: sequence of (A, B, C, plan, tile) with device tensors of one
+    dtype family (as Plan.run), ``plan`` a Plan or None (plain GEMM) and
+    ``tile`` 0 (256x128 C tiles) or 1 (128x256).  The job table is built and
+    uploaded once; ``run()`` is one persistent kernel on the stream (graph
+    capturable).  Each problem's result equals a single call's bit for bit.
+    """
+
+    def __init__(self, problems, split: bool = False, library=None):
+        self._lib = library or lib()
+        _declare_batch(self._lib)
+        self._keep = []
+        plans, index = [], {}
+        arr = (_Problem * max(len(problems), 1))()
+        dt = None
+        for i, (A, B, C, plan, tile) in enumerate(problems):
+            m, n, k = _check_operands(A, B, C)
+            d = dtype_code(A, B, C, split)
+            if dt is None:
+                dt = d
+            elif d != dt:
+                raise FmmError("all problems of a batch must share one dtype")
+            pi = -1
+            if plan is not None:
+                if id(plan) not in index:
+                    index[id(plan)] = len(plans)
+                    plans.append(plan)
+                pi = index[id(plan)]
+            arr[i] = _Problem(m, n, k, A.data_ptr(), _ld(A), B.data_ptr(), _ld(B), C.data_ptr(), _ld(C),
+                              pi, int(tile))
+            self._keep.append((A, B, C))
+        self.dtype = FMM_F64 if dt is None else dt
+        ph = (ct.c_void_p * max(len(plans), 1))(*[p.handle for p in plans])
+        h = ct.c_void_p()
+        check(self._lib.fmm_batch_create(ct.byref(h), self.dtype, len(problems), arr, len(plans), ph),
+              "fmm_batch_create")
+        self.handle = h
+        self._plans = plans
+        jobs, units = ct.c_int(), ct.c_int()
+        self._lib.fmm_batch_info(h, ct.byref(jobs), ct.byref(units))
+        self.jobs, self.units = jobs.value, units.value
+
+    def run(self, stream=None) -> None:
+        check(self._lib.fmm_batch_run(self.handle, _stream_ptr(stream)), "fmm_batch_run")
+
+    def status(self) -> int:
+        flags = ct.c_int(0)
+        self._lib.fmm_batch_status(self.handle, ct.byref(flags))
+        return int(flags.value)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.fmm_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

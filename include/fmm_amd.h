@@ -115,6 +115,35 @@ int fmm_gemm(int dtype, int m, int n, int k, const void *A, int lda,
 /* Error flags a launch left in ws (synchronises the device first). */
 int fmm_ws_status(const void *ws, int *flags);
 
+/* ---- batches: many independent problems in one persistent launch ----------
+ * No counterpart in the reference (one FMM call per entry there).  A batch is
+ * built once from a problem list (device pointers; each problem C += A*B with
+ * its own plan -- an index into plans[], or -1 for the plain GEMM -- and its
+ * own C-tile shape: 0 = 256x128, 1 = 128x256 for FMM_F32 / FMM_BF16 /
+ * FMM_F16; FMM_F64 and FMM_BF16X3 have one shape and ignore it).  Creation
+ * allocates and uploads the job table and workspace (synchronous); each run
+ * is one workspace fill and one persistent kernel on `stream`, with no
+ * allocation or copy, so it can be captured in a HIP graph.  Every problem
+ * gets exactly the result of a single fmm_run / fmm_gemm call with the same
+ * plan and dtype.  Runs of one batch must not overlap; operands must not
+ * overlap each other's C. */
+typedef struct fmm_batch_s *fmm_batch_t;
+typedef struct {
+    int m, n, k;
+    const void *A; int lda;
+    const void *B; int ldb;
+    void *C; int ldc;
+    int plan;   /* index into plans[] or -1 (plain GEMM) */
+    int tile;   /* C-tile shape, 0 or 1 */
+} fmm_problem_t;
+int fmm_batch_create(fmm_batch_t *batch, int dtype, int nprob,
+                     const fmm_problem_t *probs, int nplans,
+                     const fmm_plan_t *plans);
+int fmm_batch_run(fmm_batch_t batch, void *stream);
+int fmm_batch_info(fmm_batch_t batch, int *jobs, int *units);
+int fmm_batch_status(fmm_batch_t batch, int *flags);
+int fmm_batch_destroy(fmm_batch_t batch);
+
 /* ---- host-pointer drop-ins -------------------------------------------------
  * Copy in, run on the current GPU, copy C back; return when done.
  * fmm_host_abc replaces bl_dgemm_strassen_abc for a run-time plan. */
