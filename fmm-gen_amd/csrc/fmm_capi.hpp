@@ -287,22 +287,38 @@ void bl_dgemm_ref(int m, int n, int k, double* XA, int lda, double* XB, int ldb,
     }
 }
 
+}  // extern "C"
+
+namespace fmm {
+inline int batch_specs(int nprob, const fmm_problem_t* probs, int nplans, const fmm_plan_t* plans,
+                       std::vector<BatchSpec>* specs) {
+  if (nprob < 0 || (nprob > 0 && !probs) || nplans < 0 || (nplans > 0 && !plans)) return kEInval;
+  specs->resize((size_t)nprob);
+  for (int i = 0; i < nprob; ++i) {
+    const fmm_problem_t& p = probs[i];
+    if (p.plan < -1 || p.plan >= nplans || (p.plan >= 0 && !plans[p.plan])) return kEInval;
+    if (p.tile != 0 && p.tile != 1) return kEInval;
+    (*specs)[i] = BatchSpec{p.m, p.n, p.k, p.A, p.lda, p.B, p.ldb, p.C, p.ldc,
+                            p.plan >= 0 ? &plans[p.plan]->p : nullptr, p.tile};
+  }
+  return kOK;
+}
+}  // namespace fmm
+
+extern "C" {
+
 int fmm_batch_create(fmm_batch_t* out, int dtype, int nprob, const fmm_problem_t* probs, int nplans,
                      const fmm_plan_t* plans) {
   if (!out) return fmm::kEInval;
   *out = nullptr;
-  if (nprob < 0 || (nprob > 0 && !probs) || nplans < 0 || (nplans > 0 && !plans)) return fmm::kEInval;
-  std::vector<fmm::BatchSpec> specs((size_t)nprob);
-  for (int i = 0; i < nprob; ++i) {
-    const fmm_problem_t& p = probs[i];
-    if (p.plan < -1 || p.plan >= nplans || (p.plan >= 0 && !plans[p.plan])) return fmm::kEInval;
-    if (p.tile != 0 && p.tile != 1) return fmm::kEInval;
-    specs[i] = fmm::BatchSpec{p.m, p.n, p.k, p.A, p.lda, p.B, p.ldb, p.C, p.ldc,
-                              p.plan >= 0 ? &plans[p.plan]->p : nullptr, p.tile};
-  }
+  std::vector<fmm::BatchSpec> specs;
+  int rc = fmm::batch_specs(nprob, probs, nplans, plans, &specs);
+  if (rc != fmm::kOK) return rc;
   fmm_batch_s* b = new (std::nothrow) fmm_batch_s();
   if (!b) return fmm::kENoMem;
-  int rc = fmm::batch_build(&b->b, dtype, nprob, specs.data());
+  rc = fmm::batch_set(&b->b, dtype, nprob, specs.data(), nullptr);
+  if (rc == fmm::kOK && hipStreamSynchronize(nullptr) != hipSuccess) rc = fmm::kEHip;
+  if (rc == fmm::kOK) rc = fmm::batch_prepare(b->b);
   if (rc != fmm::kOK) {
     b->b.release();
     delete b;
@@ -310,6 +326,30 @@ int fmm_batch_create(fmm_batch_t* out, int dtype, int nprob, const fmm_problem_t
   }
   *out = b;
   return fmm::kOK;
+}
+
+int fmm_batch_set(fmm_batch_t batch, int nprob, const fmm_problem_t* probs, int nplans, const fmm_plan_t* plans,
+                  void* stream) {
+  if (!batch) return fmm::kEInval;
+  std::vector<fmm::BatchSpec> specs;
+  int rc = fmm::batch_specs(nprob, probs, nplans, plans, &specs);
+  if (rc != fmm::kOK) return rc;
+  return fmm::batch_set(&batch->b, batch->b.dtype, nprob, specs.data(), static_cast<hipStream_t>(stream));
+}
+
+int fmm_batch_reserve(fmm_batch_t batch, int jobs, int ws_ints) {
+  if (!batch || jobs < 0 || ws_ints < 0) return fmm::kEInval;
+  return fmm::batch_reserve(&batch->b, (size_t)jobs, (size_t)ws_ints);
+}
+
+int fmm_batch_generation(fmm_batch_t batch) {
+  if (!batch) return fmm::kEInval;
+  return batch->b.generation;
+}
+
+int fmm_batch_variant(fmm_batch_t batch) {
+  if (!batch) return fmm::kEInval;
+  return fmm::batch_variant(batch->b);
 }
 
 int fmm_batch_run(fmm_batch_t batch, void* stream) {

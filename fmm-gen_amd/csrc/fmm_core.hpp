@@ -92,12 +92,15 @@ struct Launch {
   int local_handover;       // 1: each queue runs on one XCD (see wait_turn)
   int pad2;
   const Job* jobs;          // device job table, or nullptr -> inl[]
+  const int* hdr;           // with a job table: {njobs, total_units} in device
+                            // memory (written before the launch), so one
+                            // captured launch serves any table contents
   int* ws;                  // workspace, layout kWs* below
   long long spin_limit;     // bounded wait, in wall_clock64() ticks
   Job inl[kInlineJobs];
 };
 
-enum : int { kErrSpin = 1, kErrPlacement = 2 };
+enum : int { kErrSpin = 1 };
 
 // Workspace layout (ints): queue x's ticket at kWsTicket*x (one 128-byte line
 // each), the error flags at kWsError, the per-tile order counters from
@@ -105,6 +108,7 @@ enum : int { kErrSpin = 1, kErrPlacement = 2 };
 constexpr int kMaxQueues = 8;
 constexpr int kWsTicket = 32;
 constexpr int kWsError = kWsTicket * kMaxQueues;
+constexpr int kWsExit = kWsError + 4;      // workgroups that finished their queue
 constexpr int kWsCounters = kWsError + 8;
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -180,9 +184,21 @@ __device__ __forceinline__ const Job& job_at(const Launch& L, int j) {
   return L.jobs ? L.jobs[j] : L.inl[j];
 }
 
-// Largest j with unit_base <= u (jobs are sorted by unit_base).
-__device__ __forceinline__ int find_job(const Launch& L, int u) {
-  int lo = 0, hi = L.njobs - 1;
+// Job count and unit total of a launch: from the device header when the jobs
+// come from a device table, else from the launch arguments.
+__device__ __forceinline__ void launch_dims(const Launch& L, int* njobs, int* total) {
+  if (L.hdr) {
+    *njobs = __builtin_amdgcn_readfirstlane(*(gcint*)&L.hdr[0]);
+    *total = __builtin_amdgcn_readfirstlane(*(gcint*)&L.hdr[1]);
+  } else {
+    *njobs = L.njobs;
+    *total = L.total_units;
+  }
+}
+
+// Largest j < njobs with unit_base <= u (jobs are sorted by unit_base).
+__device__ __forceinline__ int find_job(const Launch& L, int njobs, int u) {
+  int lo = 0, hi = njobs - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
     if (job_at(L, mid).unit_base <= u) lo = mid; else hi = mid - 1;
@@ -451,11 +467,11 @@ __device__ __forceinline__ void mfma_slice(const double (*As)[BM], const double 
 // Writer: with agent scope the release writes back the whole XCD L2
 // (buffer_wbl2 sc1), which flushes every dirty line of the XCD and measured
 // 8-13% of an FMM launch.  With local_handover all units of a tile run in the
-// queue of one XCD (queue = XCC_ID, verified by a probe at set-up and checked
-// at kernel start), so writer and reader share that L2 and the release only
-// has to wait until the writer's C stores are acknowledged by it
-// (s_waitcnt vmcnt(0)).  The counter is always accessed with agent-scope
-// atomics, which are performed at L2.
+// queue of one XCD (a workgroup serves the queue of the XCD it runs on, read
+// from HW_REG_XCC_ID; see run_units), so writer and reader share that L2 and
+// the release only has to wait until the writer's C stores are acknowledged
+// by it (s_waitcnt vmcnt(0)).  The counter is always accessed with
+// agent-scope atomics.
 __device__ __forceinline__ void acquire_c() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
 __device__ __forceinline__ void release_c(bool local) {
@@ -594,10 +610,11 @@ __device__ __forceinline__ void contract(const Job& J, const Prod& P, const Unit
 // job, all product groups over its chunk (groups in order, tiles innermost),
 // so every unit a unit waits for (same tile, earlier group) sits earlier in
 // the same queue.  Jobs in a device table (many small problems) use one queue.
-__device__ __forceinline__ bool decode_unit(const Launch& L, int x, int i, int* jo, int* qo, int* po) {
+__device__ __forceinline__ bool decode_unit(const Launch& L, int njobs, int total, int x, int i, int* jo, int* qo,
+                                            int* po) {
   if (L.jobs) {
-    if (i >= L.total_units) return false;
-    const int j = find_job(L, i);
+    if (i >= total) return false;
+    const int j = find_job(L, njobs, i);
     const Job& J = L.jobs[j];
     const int ntile = J.tiles_m * J.tiles_n;
     const int ul = i - J.unit_base;
@@ -624,6 +641,57 @@ __device__ __forceinline__ bool decode_unit(const Launch& L, int x, int i, int* 
   return false;
 }
 
+// The persistent loop of a workgroup: claim units and run them with
+// unit(j, q, p) until no unit is left.  With XCD-local C hand-over every unit
+// of a queue must run on one XCD, so a workgroup serves the queue of the XCD
+// it actually runs on (HW_REG_XCC_ID), not one derived from its index: the
+// hardware usually places workgroup i on XCD i % 8, but not always (it was
+// seen to differ after other launches), and a misplaced writer would leave C
+// in another XCD's L2.  A queue whose XCD received no workgroup is drained by
+// the last workgroup to finish its own queue; none of that queue's tiles was
+// touched in this launch, so that workgroup reads them coherently from memory.
+// The unit body has a single call site (it is inlined once).
+template <class F>
+__device__ __forceinline__ void run_units(const Launch& L, int* s_unit, F&& unit) {
+  int njobs, total;
+  launch_dims(L, &njobs, &total);
+  const int nq = L.nq;
+  int x = L.local_handover ? __builtin_amdgcn_readfirstlane(xcc_id() % nq) : (int)(blockIdx.x % (unsigned)nq);
+  const int wq = (int)(blockIdx.x / (unsigned)nq);     // index within the queue (no-ticket launches)
+  const int nw = (int)(gridDim.x / (unsigned)nq);      // workgroups per queue (no-ticket launches)
+  bool drain = false;                                  // draining the queues of XCDs without workgroups
+  int iter = 0;
+  while (true) {
+    int i;
+    if (L.use_ticket) {
+      if (threadIdx.x == 0) *s_unit = atomicAdd(L.ws + kWsTicket * x, 1);
+      __syncthreads();
+      i = __builtin_amdgcn_readfirstlane(*s_unit);
+      __syncthreads();
+    } else {
+      i = wq + iter * nw;
+    }
+    ++iter;
+    int j, q, p;
+    if (!decode_unit(L, njobs, total, x, i, &j, &q, &p)) {
+      if (!L.local_handover) break;
+      if (!drain) {
+        if (threadIdx.x == 0) *s_unit = atomicAdd(L.ws + kWsExit, 1) == (int)gridDim.x - 1 ? 1 : 0;
+        __syncthreads();
+        const int last = __builtin_amdgcn_readfirstlane(*s_unit);
+        __syncthreads();
+        if (!last) break;
+        drain = true;
+        x = 0;
+      } else if (++x >= nq) {
+        break;
+      }
+      continue;
+    }
+    unit(__builtin_amdgcn_readfirstlane(j), __builtin_amdgcn_readfirstlane(q), __builtin_amdgcn_readfirstlane(p));
+  }
+}
+
 template <int BK, int CH>
 __global__ void __launch_bounds__(NT) abc_f64_kernel(Launch L) {
   constexpr int B_KPT = TileK<BK>::B_KPT;
@@ -634,32 +702,8 @@ __global__ void __launch_bounds__(NT) abc_f64_kernel(Launch L) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WAVES_M, wn = wave / WAVES_M;
-  // Queue of this workgroup: workgroups are dispatched to the 8 XCDs round
-  // robin, so blockIdx % 8 keeps a queue's tiles (and their L2 reuse) on one
-  // XCD.  Correctness does not depend on the placement.
-  const int x = (int)(blockIdx.x % (unsigned)L.nq);
-  const int wq = (int)(blockIdx.x / (unsigned)L.nq);     // index within the queue
-  const int nw = (int)(gridDim.x / (unsigned)L.nq);      // workgroups per queue
-  if (L.local_handover && tid == 0 && xcc_id() != x) atomicOr(L.ws + kWsError, (int)kErrPlacement);
 
-  int iter = 0;
-  while (true) {
-    int i;
-    if (L.use_ticket) {
-      if (tid == 0) s_unit = atomicAdd(L.ws + kWsTicket * x, 1);
-      __syncthreads();
-      i = __builtin_amdgcn_readfirstlane(s_unit);
-      __syncthreads();
-    } else {
-      i = wq + iter * nw;
-    }
-    ++iter;
-    int j, q, p;
-    if (!decode_unit(L, x, i, &j, &q, &p)) break;
-    j = __builtin_amdgcn_readfirstlane(j);
-    q = __builtin_amdgcn_readfirstlane(q);
-    p = __builtin_amdgcn_readfirstlane(p);
-
+  run_units(L, &s_unit, [&](int j, int q, int p) {
     const Job& J = job_at(L, j);
     const int tmi = p / J.tiles_n;
     const int tni = p - tmi * J.tiles_n;
@@ -704,7 +748,7 @@ __global__ void __launch_bounds__(NT) abc_f64_kernel(Launch L) {
       scatter_c(J, P, g.row0, g.col0, lane, wm, wn, acc);
     }
     if (ordered) pass_turn(ctr, q + 1, L);
-  }
+  });
 }
 
 // --- element-wise kernels of the AB and Naive pack types --------------------
@@ -855,7 +899,8 @@ __global__ void __launch_bounds__(kEwThreads) madd_kernel(AddJob A) {
 }
 
 // Records the XCC id of each workgroup (one launch per device at set-up):
-// the host enables XCD-local hand-over only if workgroup i runs on XCC i % 8.
+// the host enables XCD-local hand-over only if the ids look like an 8-XCD part
+// (workgroup i on XCC i % 8 for a fresh launch).
 __global__ void xcc_probe_kernel(int* out) {
   if (threadIdx.x == 0) out[blockIdx.x] = xcc_id();
 }

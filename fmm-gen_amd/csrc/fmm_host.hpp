@@ -28,6 +28,12 @@ enum Status : int {
   kESpin = -5,
 };
 
+// XCD-local C hand-over between ordered products (see release_c); 0 keeps
+// the agent-scope release everywhere.
+#ifndef FMM_LOCAL_HANDOVER
+#define FMM_LOCAL_HANDOVER 1
+#endif
+
 // Partition granularity of the reference (common/include/bl_config.h:16-17):
 // the FMM core is md x kd x nd with md % (Mx*MR) == 0, nd % (Nx*NR) == 0.
 constexpr int kMR = 8;
@@ -559,7 +565,7 @@ inline int launch_jobs_with(void (*kern)(Launch), int nt, JobList JL, int* ws, h
   if (grid >= 8 * kMaxQueues) {
     grid -= grid % kMaxQueues;
     L.nq = kMaxQueues;
-    L.local_handover = JL.ordered && di.xcd_round_robin ? 1 : 0;
+    L.local_handover = FMM_LOCAL_HANDOVER && JL.ordered && di.xcd_round_robin ? 1 : 0;
   }
   hipLaunchKernelGGL(kern, dim3(grid), dim3(nt), 0, stream, L);
   if (hipGetLastError() != hipSuccess) return kEHip;

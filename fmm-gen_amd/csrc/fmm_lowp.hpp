@@ -726,33 +726,12 @@ __global__ void __launch_bounds__(C1::NT) abc_lowp_kernel(Launch L) {
   __shared__ __align__(16) char lds[LDSB];
   __shared__ int s_unit;
   const int tid = threadIdx.x;
-  const int x = (int)(blockIdx.x % (unsigned)L.nq);
-  const int wq = (int)(blockIdx.x / (unsigned)L.nq);
-  const int nw = (int)(gridDim.x / (unsigned)L.nq);
-  if (L.local_handover && tid == 0 && xcc_id() != x) atomicOr(L.ws + kWsError, (int)kErrPlacement);
-
-  int iter = 0;
-  while (true) {
-    int i;
-    if (L.use_ticket) {
-      if (tid == 0) s_unit = atomicAdd(L.ws + kWsTicket * x, 1);
-      __syncthreads();
-      i = __builtin_amdgcn_readfirstlane(s_unit);
-      __syncthreads();
-    } else {
-      i = wq + iter * nw;
-    }
-    ++iter;
-    int j, q, p;
-    if (!decode_unit(L, x, i, &j, &q, &p)) break;
-    j = __builtin_amdgcn_readfirstlane(j);
-    q = __builtin_amdgcn_readfirstlane(q);
-    p = __builtin_amdgcn_readfirstlane(p);
+  run_units(L, &s_unit, [&](int j, int q, int p) {
     const Job& J = job_at(L, j);
     const int tt = __builtin_amdgcn_readfirstlane(J.flags & kJobTileT);
     if (tt && !__is_same(C1, C2)) l_unit<C2>(L, J, q, p, lds, tid);
     else l_unit<C1>(L, J, q, p, lds, tid);
-  }
+  });
 }
 
 // The configurations built: 8 waves of 64x64 per workgroup, C tiles of

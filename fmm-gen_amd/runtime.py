@@ -31,7 +31,9 @@ STATUS = {0: "FMM_OK", -1: "FMM_EINVAL", -2: "FMM_EHIP", -3: "FMM_ENOMEM",
 EXPORTS = ("fmm_plan_create", "fmm_plan_destroy", "fmm_plan_info", "fmm_workspace_bytes",
            "fmm_workspace_bytes_pack", "fmm_run", "fmm_abc", "fmm_ab", "fmm_naive", "fmm_gemm",
            "fmm_ws_status", "fmm_host_abc", "fmm_host_run", "bl_dgemm", "bl_dgemm_ref",
-           "bl_finalize")
+           "bl_finalize", "fmm_batch_create", "fmm_batch_set", "fmm_batch_run", "fmm_batch_variant",
+           "fmm_batch_info", "fmm_batch_status", "fmm_batch_destroy", "fmm_batch_reserve",
+           "fmm_batch_generation")
 
 
 class FmmError(RuntimeError):
@@ -269,6 +271,14 @@ def _declare_batch(L):
     vp, i = ct.c_void_p, ct.c_int
     L.fmm_batch_create.argtypes = [ct.POINTER(vp), i, i, ct.POINTER(_Problem), i, ct.POINTER(vp)]
     L.fmm_batch_create.restype = i
+    L.fmm_batch_set.argtypes = [vp, i, ct.POINTER(_Problem), i, ct.POINTER(vp), vp]
+    L.fmm_batch_set.restype = i
+    L.fmm_batch_variant.argtypes = [vp]
+    L.fmm_batch_variant.restype = i
+    L.fmm_batch_reserve.argtypes = [vp, i, i]
+    L.fmm_batch_reserve.restype = i
+    L.fmm_batch_generation.argtypes = [vp]
+    L.fmm_batch_generation.restype = i
     L.fmm_batch_run.argtypes = [vp, vp]
     L.fmm_batch_run.restype = i
     L.fmm_batch_info.argtypes = [vp, ct.POINTER(i), ct.POINTER(i)]
@@ -285,16 +295,27 @@ class Batch:
 
     ``problems``: sequence of (A, B, C, plan, tile) with device tensors of one
     dtype family (as Plan.run), ``plan`` a Plan or None (plain GEMM) and
-    ``tile`` 0 (256x128 C tiles) or 1 (128x256).  The job table is built and
-    uploaded once; ``run()`` is one persistent kernel on the stream (graph
-    capturable).  Each problem's result equals a single call's bit for bit.
+    ``tile`` 0 (256x128 C tiles) or 1 (128x256).  The job table lives on the
+    device; ``run()`` is one persistent kernel on the stream (graph
+    capturable); ``set()`` replaces the problems in place with the upload on
+    a stream, so a graph captured from ``run()`` replays the new contents
+    while ``variant()`` is unchanged.  Each problem's result equals a single
+    call's bit for bit.
     """
 
-    def __init__(self, problems, split: bool = False, library=None):
+    def __init__(self, problems, split: bool = False, library=None, dtype: Optional[int] = None):
         self._lib = library or lib()
         _declare_batch(self._lib)
-        self._keep = []
-        plans, index = [], {}
+        arr, plans, ph, dt = self._encode(problems, split)
+        self.dtype = dt if dt is not None else (FMM_F64 if dtype is None else dtype)
+        h = ct.c_void_p()
+        check(self._lib.fmm_batch_create(ct.byref(h), self.dtype, len(problems), arr, len(plans), ph),
+              "fmm_batch_create")
+        self.handle = h
+        self._info()
+
+    def _encode(self, problems, split):
+        keep, plans, index = [], [], {}
         arr = (_Problem * max(len(problems), 1))()
         dt = None
         for i, (A, B, C, plan, tile) in enumerate(problems):
@@ -312,17 +333,36 @@ class Batch:
                 pi = index[id(plan)]
             arr[i] = _Problem(m, n, k, A.data_ptr(), _ld(A), B.data_ptr(), _ld(B), C.data_ptr(), _ld(C),
                               pi, int(tile))
-            self._keep.append((A, B, C))
-        self.dtype = FMM_F64 if dt is None else dt
+            keep.append((A, B, C))
         ph = (ct.c_void_p * max(len(plans), 1))(*[p.handle for p in plans])
-        h = ct.c_void_p()
-        check(self._lib.fmm_batch_create(ct.byref(h), self.dtype, len(problems), arr, len(plans), ph),
-              "fmm_batch_create")
-        self.handle = h
+        self._keep = keep
         self._plans = plans
+        return arr, plans, ph, dt
+
+    def _info(self):
         jobs, units = ct.c_int(), ct.c_int()
-        self._lib.fmm_batch_info(h, ct.byref(jobs), ct.byref(units))
+        self._lib.fmm_batch_info(self.handle, ct.byref(jobs), ct.byref(units))
         self.jobs, self.units = jobs.value, units.value
+
+    def set(self, problems, split: bool = False, stream=None) -> None:
+        """Replace the problems (same dtype); the previous run must be done."""
+        arr, plans, ph, dt = self._encode(problems, split)
+        if dt is not None and dt != self.dtype:
+            raise FmmError("a batch keeps its dtype")
+        check(self._lib.fmm_batch_set(self.handle, len(problems), arr, len(plans), ph, _stream_ptr(stream)),
+              "fmm_batch_set")
+        self._info()
+
+    def variant(self) -> int:
+        return int(self._lib.fmm_batch_variant(self.handle))
+
+    def generation(self) -> int:
+        """Changes whenever the device buffers are reallocated (captured
+        graphs of run() are then stale)."""
+        return int(self._lib.fmm_batch_generation(self.handle))
+
+    def reserve(self, jobs: int, ws_ints: int) -> None:
+        check(self._lib.fmm_batch_reserve(self.handle, int(jobs), int(ws_ints)), "fmm_batch_reserve")
 
     def run(self, stream=None) -> None:
         check(self._lib.fmm_batch_run(self.handle, _stream_ptr(stream)), "fmm_batch_run")

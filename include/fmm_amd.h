@@ -117,16 +117,21 @@ int fmm_ws_status(const void *ws, int *flags);
 
 /* ---- batches: many independent problems in one persistent launch ----------
  * No counterpart in the reference (one FMM call per entry there).  A batch is
- * built once from a problem list (device pointers; each problem C += A*B with
- * its own plan -- an index into plans[], or -1 for the plain GEMM -- and its
- * own C-tile shape: 0 = 256x128, 1 = 128x256 for FMM_F32 / FMM_BF16 /
- * FMM_F16; FMM_F64 and FMM_BF16X3 have one shape and ignore it).  Creation
- * allocates and uploads the job table and workspace (synchronous); each run
- * is one workspace fill and one persistent kernel on `stream`, with no
- * allocation or copy, so it can be captured in a HIP graph.  Every problem
- * gets exactly the result of a single fmm_run / fmm_gemm call with the same
- * plan and dtype.  Runs of one batch must not overlap; operands must not
- * overlap each other's C. */
+ * built from a problem list (device pointers; each problem C += A*B with its
+ * own plan -- an index into plans[], or -1 for the plain GEMM -- and its own
+ * C-tile shape: 0 = 256x128, 1 = 128x256 for FMM_F32 / FMM_BF16 / FMM_F16;
+ * FMM_F64 and FMM_BF16X3 have one shape and ignore it).  The batch owns its
+ * device job table and workspace (grow-only).  fmm_batch_create builds and
+ * uploads synchronously (and prepares every kernel variant of the dtype, so
+ * no later run needs device set-up); fmm_batch_set rebuilds in place with the upload
+ * ordered on `stream` (the batch's previous run must have completed).  Each
+ * run is one workspace fill and one persistent kernel over all CUs on
+ * `stream`, with no allocation or copy; its launch arguments depend only on
+ * the batch's buffers, so a HIP graph captured from fmm_batch_run replays any
+ * later contents of the same fmm_batch_variant.  Every problem gets exactly
+ * the result of a single fmm_run / fmm_gemm call with the same plan and
+ * dtype.  Runs of one batch must not overlap; no problem's C may overlap
+ * another's operands. */
 typedef struct fmm_batch_s *fmm_batch_t;
 typedef struct {
     int m, n, k;
@@ -139,7 +144,17 @@ typedef struct {
 int fmm_batch_create(fmm_batch_t *batch, int dtype, int nprob,
                      const fmm_problem_t *probs, int nplans,
                      const fmm_plan_t *plans);
+int fmm_batch_set(fmm_batch_t batch, int nprob, const fmm_problem_t *probs,
+                  int nplans, const fmm_plan_t *plans, void *stream);
 int fmm_batch_run(fmm_batch_t batch, void *stream);
+/* Kernel variant (0..3) the batch's current contents need. */
+int fmm_batch_variant(fmm_batch_t batch);
+/* Grow the batch's device buffers to `jobs` jobs and `ws_ints` workspace
+ * ints now.  Every reallocation (here or in fmm_batch_set) increments the
+ * batch's generation; a graph captured from fmm_batch_run is valid while the
+ * generation and the variant are unchanged. */
+int fmm_batch_reserve(fmm_batch_t batch, int jobs, int ws_ints);
+int fmm_batch_generation(fmm_batch_t batch);
 int fmm_batch_info(fmm_batch_t batch, int *jobs, int *units);
 int fmm_batch_status(fmm_batch_t batch, int *flags);
 int fmm_batch_destroy(fmm_batch_t batch);

@@ -81,3 +81,18 @@ def test_bl_dgemm_ref_is_textbook(rt):
     dp = ct.POINTER(ct.c_double)
     rt.lib().bl_dgemm_ref(m, n, k, A.ctypes.data_as(dp), m, B.ctypes.data_as(dp), k, C.ctypes.data_as(dp), m)
     assert np.array_equal(C, want)
+
+
+def test_batch_argument_validation_before_device(fmm, rt):
+    lib = rt.lib()
+    rt._declare_batch(lib)
+    h = ct.c_void_p()
+    probs = (rt._Problem * 1)(rt._Problem(8, 8, 8, None, 8, None, 8, None, 8, 3, 0))
+    plans = (ct.c_void_p * 1)()
+    assert lib.fmm_batch_create(ct.byref(h), 1, 1, probs, 0, plans) == -1      # plan index out of range
+    probs[0].plan, probs[0].tile = -1, 2
+    assert lib.fmm_batch_create(ct.byref(h), 1, 1, probs, 0, plans) == -1      # tile shape 0 or 1
+    probs[0].tile = 0
+    assert lib.fmm_batch_create(ct.byref(h), 9, 1, probs, 0, plans) == -4      # unknown dtype
+    assert lib.fmm_batch_create(ct.byref(h), 1, -1, probs, 0, plans) == -1
+    assert lib.fmm_batch_run(None, None) == -1

@@ -52,7 +52,13 @@ def run(torch, rt, dtype, A, B, C, plan=None, ldpad=0):
     if plan is None:
         rt.gemm(dA, dB, dC, split=split)
     else:
-        plan.abc(dA, dB, dC, split=split)
+        # at least the ticket and error-flag slots (264 ints) even when the
+        # shape needs no ordering, so the flag read below stays in bounds
+        ws = torch.zeros(max(plan.workspace_bytes(m, n, k) // 4, 264) + 64, dtype=torch.int32, device="cuda")
+        plan.abc(dA, dB, dC, ws=ws, split=split)
+        torch.cuda.synchronize()
+        flags = rt.ws_status(ws)
+        assert flags == 0, f"launch error flags {flags} ({dtype}, {m}x{n}x{k})"
     torch.cuda.synchronize()
     return dC.cpu().numpy().astype(np.float32)
 
