@@ -1,5 +1,5 @@
 """Benchmark: effective GFLOPS (2mnk/t) of the best generated ABC FMM vs the
-project's plain FP64 MFMA GEMM, on one MI355X.
+project's plain MFMA GEMM, on one MI355X.
 
 One *step* is one C += A*B call (FP64, column-major, inputs resident in HBM)
 through the C-ABI entry fmm_abc.  Default workload: the one-level <3,3,3>
@@ -11,6 +11,21 @@ plus the plain GEMM's rate on the same inputs, the sampled max-norm error of
 both against an extended-precision dot product, a roofline object for the
 FMM kernel and a CPU baseline (the oracle's C restatement of the reference
 ABC algorithm, timed on the host cores on a bounded sample).
+
+Further sections (``--sections``; all by default), reported beside the
+headline in the same line:
+  precision  8192^3 from FP64 U[-1,1) inputs: plain GEMM and the <2,2,2> L1,
+             <2,2,2> L2 and <3,2,3> L1 FMMs with FP32, BF16, FP16 and
+             split-BF16 products (FP32 accumulation), and FP64 refinement
+             (fmm_refine); effective TFLOP/s and relative error against the
+             FP64 product of the original inputs (sampled entries);
+  batched    1024 FP32 problems, m, n, k log-uniform in [192, 4096], one
+             persistent launch with per-problem choices of the autotuner
+             (calibrated on the device first) against all-GEMM batches and
+             single calls;
+  serving    Poisson arrivals of the GEMM request mix (60% square, 30%
+             rank-k, 10% tall-skinny; 70% FP32, 30% FP64) under a 200 GB HBM
+             budget: p50 / p99 latency and throughput.
 """
 import argparse
 import json
@@ -46,6 +61,10 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-gemm", action="store_true", help="skip the plain-GEMM comparison")
+    ap.add_argument("--sections", default="headline,precision,batched,serving",
+                    help="comma list of headline, precision, batched, serving")
+    ap.add_argument("--qps", type=float, default=800.0, help="serving: arrival rate")
+    ap.add_argument("--requests", type=int, default=1500, help="serving: number of requests")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB three times (HBM counter calibration)")
     return ap.parse_args()
@@ -127,6 +146,157 @@ def cpu_baseline(F, algs, levels):
                       f"levels {levels}, m=n=k={s}, FP64, seeded U[-1,1), min of 3 runs"}
 
 
+def rel_sampled(A, B, C0, C, nsamp=512, seed=3):
+    """Relative error ||C - C_exact|| / ||C_exact|| over sampled entries, with
+    C_exact from extended-precision dot products of the FP64 inputs."""
+    import numpy as np
+    import torch
+    m, k = A.shape
+    n = B.shape[1]
+    g = np.random.default_rng(seed)
+    ii = g.integers(0, m, nsamp)
+    jj = g.integers(0, n, nsamp)
+    It = torch.as_tensor(ii, device=A.device)
+    Jt = torch.as_tensor(jj, device=A.device)
+    rows = A[It, :].cpu().numpy().astype(np.longdouble)
+    cols = B[:, Jt].cpu().numpy().astype(np.longdouble)
+    ex = C0[It, Jt].cpu().numpy().astype(np.longdouble) + (rows * cols.T).sum(axis=1)
+    got = C[It, Jt].cpu().numpy().astype(np.longdouble)
+    return float(np.sqrt(((got - ex) ** 2).sum() / (ex ** 2).sum()))
+
+
+def ev_ms(fn, reps=3):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def section_precision(F, rt, seed):
+    """Config 'precision sweep': 8192^3, FP64 U[-1,1) inputs cast per dtype."""
+    import torch
+    N = 8192
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    A64 = rt.colmajor(N, N)
+    B64 = rt.colmajor(N, N)
+    A64.uniform_(-1.0, 1.0, generator=gen)
+    B64.uniform_(-1.0, 1.0, generator=gen)
+    Z64 = rt.colmajor(N, N).zero_()
+    specs = {"gemm": None, "222L1": (["222"], [1]), "222L2": (["222"], [2]), "323L1": (["323"], [1])}
+    plans = {a: (rt.Plan(F.build_algorithm(*sp)) if sp else None) for a, sp in specs.items()}
+    fl = 2.0 * N ** 3
+    rows = []
+    C = rt.colmajor(N, N, dtype=torch.float32)
+    for dt in ("f32", "bf16", "f16", "bf16x3"):
+        ti = {"f32": torch.float32, "bf16": torch.bfloat16, "f16": torch.float16, "bf16x3": torch.float32}[dt]
+        A = rt.as_colmajor(A64.to(ti))
+        B = rt.as_colmajor(B64.to(ti))
+        split = dt == "bf16x3"
+        for a, plan in plans.items():
+            def call():
+                if plan is None:
+                    rt.gemm(A, B, C, split=split)
+                else:
+                    plan.abc(A, B, C, split=split)
+            C.zero_()
+            call()
+            torch.cuda.synchronize()
+            err = rel_sampled(A64, B64, Z64, C)
+            ms = ev_ms(call)
+            rows.append({"dtype": dt, "alg": a, "ms": round(ms, 3), "eff_tflops": round(fl / ms / 1e9, 1),
+                         "rel_err": float(f"{err:.3e}")})
+        del A, B
+    C64 = rt.colmajor(N, N)
+    for main, a in (("bf16", "gemm"), ("bf16", "222L1"), ("f16", "gemm"), ("f32", "222L1"), ("bf16x3", "222L1")):
+        def call():
+            rt.refine(A64, B64, C64, main=main, main_plan=plans[a], corr="f32")
+        C64.zero_()
+        call()
+        torch.cuda.synchronize()
+        err = rel_sampled(A64, B64, Z64, C64)
+        ms = ev_ms(call, 2)
+        rows.append({"dtype": f"{main}+refine(f32)", "alg": a, "ms": round(ms, 3),
+                     "eff_tflops": round(fl / ms / 1e9, 1), "rel_err": float(f"{err:.3e}")})
+    # best FMM vs the same dtype's plain GEMM (both within that dtype's stated
+    # accuracy, tests/tolerance.py)
+    best = {}
+    for dt in ("f32", "bf16", "f16", "bf16x3"):
+        g = next(r for r in rows if r["dtype"] == dt and r["alg"] == "gemm")
+        f = max((r for r in rows if r["dtype"] == dt and r["alg"] != "gemm"), key=lambda r: r["eff_tflops"])
+        best[dt] = {"best_fmm": f["alg"], "fmm_tflops": f["eff_tflops"], "gemm_tflops": g["eff_tflops"],
+                    "speedup": round(f["eff_tflops"] / g["eff_tflops"], 3)}
+    return {"shape": [N, N, N], "inputs": "FP64 U[-1,1) cast per dtype, C = 0",
+            "err_metric": "relative Frobenius error vs the FP64 product of the FP64 inputs, 512 sampled entries",
+            "rows": rows, "best_vs_gemm": best}
+
+
+def section_batched(F, rt, seed):
+    """Config 'batched': 1024 FP32 problems in one persistent launch."""
+    import numpy as np
+    import torch
+    from collections import Counter
+    from fmm_gen_amd import autotune as AT
+    tuner = AT.Tuner("f32")
+
+    def device_time(m, n, k, alg, tile, copies):
+        A = rt.colmajor(m, k, dtype=torch.float32).uniform_(-1, 1)
+        B = rt.colmajor(k, n, dtype=torch.float32).uniform_(-1, 1)
+        Cs = [rt.colmajor(m, n, dtype=torch.float32).zero_() for _ in range(copies)]
+        b, _ = tuner.batch([(A, B, C) for C in Cs], choices=[(alg, tile)] * copies)
+        t = ev_ms(b.run) / 1e3
+        b.close()
+        return t
+    t0 = time.perf_counter()
+    theta = tuner.calibrate(device_time)
+    tcal = time.perf_counter() - t0
+    rng = np.random.default_rng(seed)
+    dims = np.exp(rng.uniform(np.log(192), np.log(4096), (1024, 3))).round().astype(int)
+    ops = []
+    for m, n, k in dims:
+        A = rt.colmajor(int(m), int(k), dtype=torch.float32).uniform_(-1, 1)
+        B = rt.colmajor(int(k), int(n), dtype=torch.float32).uniform_(-1, 1)
+        C = rt.colmajor(int(m), int(n), dtype=torch.float32).uniform_(-1, 1)
+        ops.append((A, B, C))
+    fl = float(sum(2.0 * m * n * k for m, n, k in dims))
+    tuned, choices = tuner.batch(ops)
+    t_tuned = ev_ms(tuned.run)
+    assert tuned.status() == 0
+    gch = [("gemm", min(range(2), key=lambda t: tuner.model.cost(tuner.features(A.shape[0], B.shape[1], A.shape[1],
+                                                                               "gemm", t)))) for A, B, C in ops]
+    gb, _ = tuner.batch(ops, choices=gch)
+    t_gemm = ev_ms(gb.run)
+
+    def singles():
+        for A, B, C in ops:
+            rt.gemm(A, B, C)
+    t_single = ev_ms(singles, 1)
+    return {"problems": len(ops), "tflop": round(fl / 1e12, 3), "dtype": "f32",
+            "tuned_ms": round(t_tuned, 3), "tuned_tflops": round(fl / t_tuned / 1e9, 2),
+            "gemm_batch_ms": round(t_gemm, 3), "gemm_batch_tflops": round(fl / t_gemm / 1e9, 2),
+            "gemm_single_calls_ms": round(t_single, 3),
+            "choices": {f"{a}/tile{t}": c for (a, t), c in Counter(map(tuple, choices)).items()},
+            "model_theta": [float(f"{x:.4g}") for x in theta], "calibration_s": round(tcal, 1)}
+
+
+def section_serving(F, rt, seed, qps, nreq):
+    """Config 'serving': Poisson arrivals of the request mix."""
+    from fmm_gen_amd import serve as S
+    ex = S.DeviceExecutor(use_graphs=True, seed=seed)
+    srv = S.Server(ex, hbm_budget=200e9, lanes_per_dtype=2, max_batch=256)
+    ex.warmup(srv.lanes)
+    srv.serve(S.poisson_mix(nreq, qps=qps, seed=seed), timeout=600)
+    st = srv.stats()
+    st.update({"qps": qps, "hbm_budget_gb": 200, "graph_replays": ex.replays, "graph_captures": ex.captures})
+    return st
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -140,6 +310,20 @@ def main():
     from __graft_entry__ import load_package
     F = load_package()
     from fmm_gen_amd import runtime as rt
+    sections = set(args.sections.split(","))
+    extra = {}
+    if "precision" in sections:
+        extra["precision_sweep"] = section_precision(F, rt, args.seed)
+        torch.cuda.empty_cache()
+    if "batched" in sections:
+        extra["batched"] = section_batched(F, rt, args.seed)
+        torch.cuda.empty_cache()
+    if "serving" in sections:
+        extra["serving"] = section_serving(F, rt, args.seed, args.qps, args.requests)
+        torch.cuda.empty_cache()
+    if "headline" not in sections:
+        print(json.dumps(extra), flush=True)
+        return
 
     algs, levels, m, n, k = WORKLOADS[args.workload]
     alg = F.build_algorithm(algs, levels)
@@ -231,6 +415,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

@@ -6,6 +6,7 @@
 #include <cstdlib>
 
 #include "fmm_batch.hpp"
+#include "fmm_refine.hpp"
 #include "fmm_host.hpp"
 #include "fmm_amd.h"
 
@@ -61,10 +62,11 @@ inline int run_fmm(Plan* P, int pack, int dtype, int m, int n, int k, const void
                    int ldb, void* C, int ldc, void* ws, size_t ws_bytes, hipStream_t s) {
   if (pack != kPackABC && pack != kPackAB && pack != kPackNaive) return kEInval;
   if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
-  if (pack != kPackABC && dtype != kF64) return kEUnsupported;
+  KGeo G;
+  if (!dtype_geo(dtype, &G)) return kEUnsupported;
   if (m == 0 || n == 0 || k == 0) return kOK;
   PackLayout Lo;
-  if (pack == kPackABC || !pack_layout(*P, pack, m, n, k, &Lo))
+  if (pack == kPackABC || !pack_layout(*P, pack, m, n, k, &Lo, G.sin, G.sc))
     return run_abc(P, dtype, m, n, k, A, lda, B, ldb, C, ldc, ws, ws_bytes, s);
   void* w = ws;
   bool own = false;
@@ -74,8 +76,7 @@ inline int run_fmm(Plan* P, int pack, int dtype, int m, int n, int k, const void
   } else if (ws_bytes < Lo.bytes) {
     return kEInval;
   }
-  int rc = launch_pack(P, pack, Lo, m, n, k, static_cast<const double*>(A), lda, static_cast<const double*>(B), ldb,
-                       static_cast<double*>(C), ldc, w, s);
+  int rc = launch_pack(P, pack, dtype, Lo, m, n, k, A, lda, B, ldb, C, ldc, w, s);
   if (own) (void)hipFreeAsync(w, s);
   return rc;
 }
@@ -378,6 +379,100 @@ int fmm_batch_destroy(fmm_batch_t batch) {
   (void)hipSetDevice(cur);
   delete batch;
   return fmm::kOK;
+}
+
+}  // extern "C"
+
+namespace fmm {
+inline bool refine_types_ok(int main_dtype, int corr_dtype) {
+  return (main_dtype == kF32 || main_dtype == kBF16 || main_dtype == kF16 || main_dtype == kBF16x3) &&
+         (corr_dtype == kF32 || corr_dtype == kBF16x3);
+}
+
+inline size_t refine_ws(fmm_plan_t mp, int main_dtype, fmm_plan_t cp, int corr_dtype, int m, int n, int k,
+                        RefLayout* out) {
+  const size_t wm = mp ? workspace_bytes(&mp->p, kPackABC, m, n, k) : 0;
+  const size_t wc = cp ? workspace_bytes(&cp->p, kPackABC, m, n, 2 * k) : 0;
+  *out = ref_layout(main_dtype, m, n, k, wm, wc);
+  (void)corr_dtype;
+  return out->bytes;
+}
+}  // namespace fmm
+
+extern "C" {
+
+size_t fmm_refine_workspace_bytes(fmm_plan_t main_plan, int main_dtype, fmm_plan_t corr_plan, int corr_dtype, int m,
+                                  int n, int k) {
+  if (m <= 0 || n <= 0 || k <= 0 || !fmm::refine_types_ok(main_dtype, corr_dtype)) return 0;
+  fmm::RefLayout L;
+  return fmm::refine_ws(main_plan, main_dtype, corr_plan, corr_dtype, m, n, k, &L);
+}
+
+int fmm_refine(fmm_plan_t main_plan, int main_dtype, fmm_plan_t corr_plan, int corr_dtype, int m, int n, int k,
+               const double* A, int lda, const double* B, int ldb, double* C, int ldc, void* ws, size_t ws_bytes,
+               void* stream) {
+  using namespace fmm;
+  if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
+  if (!refine_types_ok(main_dtype, corr_dtype)) return kEUnsupported;
+  if (m == 0 || n == 0 || k == 0) return kOK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  RefLayout Lo;
+  const size_t need = refine_ws(main_plan, main_dtype, corr_plan, corr_dtype, m, n, k, &Lo);
+  char* w = static_cast<char*>(ws);
+  bool own = false;
+  if (!w) {
+    if (hipMallocAsync(reinterpret_cast<void**>(&w), need, s) != hipSuccess) return kENoMem;
+    own = true;
+  } else if (ws_bytes < need) {
+    return kEInval;
+  }
+  float* AA = reinterpret_cast<float*>(w + Lo.aa);
+  float* BB = reinterpret_cast<float*>(w + Lo.bb);
+  float* Ct = reinterpret_cast<float*>(w + Lo.ct);
+  float* Et = reinterpret_cast<float*>(w + Lo.et);
+  const long long mk = (long long)m * k, kn = (long long)k * n, mn = (long long)Lo.lt * n;
+  int rc = kOK;
+  auto chk = [&]() {
+    if (rc == kOK && hipGetLastError() != hipSuccess) rc = kEHip;
+  };
+  switch (main_dtype) {
+    case kBF16:
+      split_a_kernel<kInBF16><<<ew_blocks(mk), kRefThreads, 0, s>>>(m, k, A, lda, reinterpret_cast<u16*>(w + Lo.a1),
+                                                                    Lo.la1, AA, Lo.laa);
+      split_b_kernel<kInBF16><<<ew_blocks(kn), kRefThreads, 0, s>>>(k, n, B, ldb, reinterpret_cast<u16*>(w + Lo.b1),
+                                                                    Lo.lb1, BB, Lo.lbb);
+      break;
+    case kF16:
+      split_a_kernel<kInF16><<<ew_blocks(mk), kRefThreads, 0, s>>>(m, k, A, lda, reinterpret_cast<u16*>(w + Lo.a1),
+                                                                   Lo.la1, AA, Lo.laa);
+      split_b_kernel<kInF16><<<ew_blocks(kn), kRefThreads, 0, s>>>(k, n, B, ldb, reinterpret_cast<u16*>(w + Lo.b1),
+                                                                   Lo.lb1, BB, Lo.lbb);
+      break;
+    default:   // FP32 inputs (FMM_F32, FMM_BF16X3)
+      split_a_kernel<kInF32><<<ew_blocks(mk), kRefThreads, 0, s>>>(m, k, A, lda, reinterpret_cast<float*>(w + Lo.a1),
+                                                                   Lo.la1, AA, Lo.laa);
+      split_b_kernel<kInF32><<<ew_blocks(kn), kRefThreads, 0, s>>>(k, n, B, ldb, reinterpret_cast<float*>(w + Lo.b1),
+                                                                   Lo.lb1, BB, Lo.lbb);
+      break;
+  }
+  chk();
+  zero_f32_kernel<<<ew_blocks(mn), kRefThreads, 0, s>>>(Ct, mn);
+  zero_f32_kernel<<<ew_blocks(mn), kRefThreads, 0, s>>>(Et, mn);
+  chk();
+  if (rc == kOK)
+    rc = main_plan ? run_abc(&main_plan->p, main_dtype, m, n, k, w + Lo.a1, (int)Lo.la1, w + Lo.b1, (int)Lo.lb1, Ct,
+                             (int)Lo.lt, w + Lo.wsm, Lo.wsc - Lo.wsm, s)
+                   : run_gemm(main_dtype, m, n, k, w + Lo.a1, (int)Lo.la1, w + Lo.b1, (int)Lo.lb1, Ct, (int)Lo.lt, s);
+  if (rc == kOK)
+    rc = corr_plan ? run_abc(&corr_plan->p, corr_dtype, m, n, 2 * k, AA, (int)Lo.laa, BB, (int)Lo.lbb, Et,
+                             (int)Lo.lt, w + Lo.wsc, Lo.bytes - Lo.wsc, s)
+                   : run_gemm(corr_dtype, m, n, 2 * k, AA, (int)Lo.laa, BB, (int)Lo.lbb, Et, (int)Lo.lt, s);
+  if (rc == kOK) {
+    combine_kernel<<<ew_blocks((long long)m * n), kRefThreads, 0, s>>>(m, n, Ct, Et, Lo.lt, C, ldc);
+    chk();
+  }
+  if (own) (void)hipFreeAsync(w, s);
+  return rc;
 }
 
 void bl_finalize(void) {

@@ -668,7 +668,9 @@ inline size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Workspace layout of an AB / Naive call; false when the shape has no FMM
 // core (then the call is the ABC path's plain GEMMs).
-inline bool pack_layout(const Plan& P, int pack, int m, int n, int k, PackLayout* L) {
+// Element sizes: sin of A / B (and of the S, T sums), sc of C (and of M).
+// The default (FP64) layout is the largest, so its size bounds every dtype's.
+inline bool pack_layout(const Plan& P, int pack, int m, int n, int k, PackLayout* L, int sin = 8, int sc = 8) {
   PackLayout o;
   o.md = m - m % (P.Mx * kMR);
   o.kd = k - k % P.Kx;
@@ -681,15 +683,15 @@ inline bool pack_layout(const Plan& P, int pack, int m, int n, int k, PackLayout
   o.nlive = P.R + (o.kr > 0 ? P.Mx * P.Nx : 0);
   o.ldm = o.ms;                        // ms % 8 == 0: 16-byte columns
   o.lds = o.ms;
-  o.ldt = o.ks + (o.ks & 1);           // even, so T columns stay 16-byte aligned
+  o.ldt = (o.ks + 7) & ~7;             // T columns stay 16-byte aligned for every type
   size_t off = 0;
   o.m_off = off;
-  off += round256(sizeof(double) * (size_t)o.ldm * o.ns * o.nlive);
+  off += round256((size_t)sc * (size_t)o.ldm * o.ns * o.nlive);
   if (pack == kPackNaive) {
     o.s_off = off;
-    off += round256(sizeof(double) * (size_t)o.lds * o.ks * P.R);
+    off += round256((size_t)sin * (size_t)o.lds * o.ks * P.R);
     o.t_off = off;
-    off += round256(sizeof(double) * (size_t)o.ldt * o.ns * P.R);
+    off += round256((size_t)sin * (size_t)o.ldt * o.ns * P.R);
   }
   o.bytes = off;
   *L = o;
@@ -700,10 +702,13 @@ inline int ew_grid(int rows, int cols) { return tiles(rows, kEwRows) * tiles(col
 
 // One AB or Naive call C += A*B on `stream`: [presum A, presum B,] one product
 // launch (the products into M, the k-fringe products into M, the right and
-// bottom fringe GEMMs straight into C), then M_Add into the C core.
-inline int launch_pack(Plan* P, int pack, const PackLayout& Lo, int m, int n, int k, const double* A,
-                       long long lda, const double* B, long long ldb, double* C, long long ldc, void* ws,
+// bottom fringe GEMMs straight into C), then M_Add into the C core.  Lo must
+// be the layout of this dtype (pack_layout with its element sizes).
+inline int launch_pack(Plan* P, int pack, int dtype, const PackLayout& Lo, int m, int n, int k, const void* A,
+                       long long lda, const void* B, long long ldb, void* C, long long ldc, void* ws,
                        hipStream_t stream) {
+  KGeo G;
+  if (!dtype_geo(dtype, &G)) return kEUnsupported;
   Prod *dp, *xp, *ip;
   Term *dt, *xt, *it;
   int rc = device_tables(P, P->main, &dp, &dt);
@@ -714,50 +719,82 @@ inline int launch_pack(Plan* P, int pack, const PackLayout& Lo, int m, int n, in
   if (rc == kOK) rc = device_clists(P, &cptr, &cprod, &cw);
   if (rc != kOK) return rc;
   char* w = static_cast<char*>(ws);
-  double* M = reinterpret_cast<double*>(w + Lo.m_off);
-  const double* pa = A;
-  const double* pb = B;
-  long long la = lda, lb = ldb;
-  int b_even = (Lo.ks & 1) == 0;
+  void* M = w + Lo.m_off;
+  const void* pa = A;
+  const void* pb = B;
+  long long la = lda, lb = ldb, kblk = Lo.ks;
   if (pack == kPackNaive) {
-    double* S = reinterpret_cast<double*>(w + Lo.s_off);
-    double* T = reinterpret_cast<double*>(w + Lo.t_off);
-    SumJob sa{};
-    sa.X = A;
-    sa.Y = S;
-    sa.ldx = lda;
-    sa.ldy = Lo.lds;
-    sa.rows = Lo.ms;
-    sa.cols = Lo.ks;
-    sa.ny = 1;
-    sa.side = 0;
-    sa.vec = ((uintptr_t)A & 15) == 0 && (lda & 1) == 0 && ((uintptr_t)S & 15) == 0;
-    sa.prods = dp;
-    sa.terms = dt;
-    presum_kernel<<<dim3(ew_grid(Lo.ms, Lo.ks), P->R), dim3(kEwThreads), 0, stream>>>(sa);
-    if (hipGetLastError() != hipSuccess) return kEHip;
-    SumJob sb = sa;
-    sb.X = B;
-    sb.Y = T;
-    sb.ldx = ldb;
-    sb.ldy = Lo.ldt;
-    sb.rows = Lo.ks;
-    sb.cols = Lo.ns;
-    sb.side = 1;
-    sb.vec = ((uintptr_t)B & 15) == 0 && (ldb & 1) == 0 && b_even && ((uintptr_t)T & 15) == 0;
-    presum_kernel<<<dim3(ew_grid(Lo.ks, Lo.ns), P->R), dim3(kEwThreads), 0, stream>>>(sb);
+    void* S = w + Lo.s_off;
+    void* T = w + Lo.t_off;
+    const dim3 ga(ew_grid(Lo.ms, Lo.ks), P->R), gb(ew_grid(Lo.ks, Lo.ns), P->R);
+    if (dtype == kF64) {
+      SumJob sa{};
+      sa.X = static_cast<const double*>(A);
+      sa.Y = static_cast<double*>(S);
+      sa.ldx = lda;
+      sa.ldy = Lo.lds;
+      sa.rows = Lo.ms;
+      sa.cols = Lo.ks;
+      sa.ny = 1;
+      sa.side = 0;
+      sa.vec = ((uintptr_t)A & 15) == 0 && (lda & 1) == 0 && ((uintptr_t)S & 15) == 0;
+      sa.prods = dp;
+      sa.terms = dt;
+      presum_kernel<<<ga, dim3(kEwThreads), 0, stream>>>(sa);
+      if (hipGetLastError() != hipSuccess) return kEHip;
+      SumJob sb = sa;
+      sb.X = static_cast<const double*>(B);
+      sb.Y = static_cast<double*>(T);
+      sb.ldx = ldb;
+      sb.ldy = Lo.ldt;
+      sb.rows = Lo.ks;
+      sb.cols = Lo.ns;
+      sb.side = 1;
+      sb.vec = ((uintptr_t)B & 15) == 0 && (ldb & 1) == 0 && (Lo.ks & 1) == 0 && ((uintptr_t)T & 15) == 0;
+      presum_kernel<<<gb, dim3(kEwThreads), 0, stream>>>(sb);
+    } else {
+      SumJobL sa{};
+      sa.X = A;
+      sa.Y = S;
+      sa.ldx = lda;
+      sa.ldy = Lo.lds;
+      sa.rows = Lo.ms;
+      sa.cols = Lo.ks;
+      sa.ny = 1;
+      sa.side = 0;
+      sa.prods = dp;
+      sa.terms = dt;
+      SumJobL sb = sa;
+      sb.X = B;
+      sb.Y = T;
+      sb.ldx = ldb;
+      sb.ldy = Lo.ldt;
+      sb.rows = Lo.ks;
+      sb.cols = Lo.ns;
+      sb.side = 1;
+      if (dtype == kBF16) {
+        presum_lowp_kernel<kInBF16><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
+        presum_lowp_kernel<kInBF16><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
+      } else if (dtype == kF16) {
+        presum_lowp_kernel<kInF16><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
+        presum_lowp_kernel<kInF16><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
+      } else {
+        presum_lowp_kernel<kInF32><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
+        presum_lowp_kernel<kInF32><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
+      }
+    }
     if (hipGetLastError() != hipSuccess) return kEHip;
     pa = S;
     la = Lo.lds;
     pb = T;
     lb = Lo.ldt;
-    b_even = 1;   // T_r starts at column r*ns of an even-ld matrix
+    kblk = 0;   // T_r is block (0, r): no row offset
   }
   JobList JL;
-  add_gemm_job(JL, ip, it, m, n - Lo.nd, k, A, lda, B + (long long)Lo.nd * ldb, ldb, C + (long long)Lo.nd * ldc,
-               ldc);
-  add_gemm_job(JL, ip, it, m - Lo.md, Lo.nd, k, A + Lo.md, lda, B, ldb, C + Lo.md, ldc);
-  auto add_m_job = [&](const double* a, long long ja, const double* b, long long jb, int even, int kr, int nprod,
+  add_gemm_job(JL, G, ip, it, m, n - Lo.nd, k, A, lda, eoff(B, (long long)Lo.nd * ldb, G.sin), ldb,
+               eoff(C, (long long)Lo.nd * ldc, G.sc), ldc);
+  add_gemm_job(JL, G, ip, it, m - Lo.md, Lo.nd, k, eoff(A, Lo.md, G.sin), lda, B, ldb, eoff(C, Lo.md, G.sc), ldc);
+  auto add_m_job = [&](const void* a, long long ja, const void* b, long long jb, long long kb, int kr, int nprod,
                        const Prod* prods, const Term* terms) {
     Job j{};
     j.A = a;
@@ -770,38 +807,57 @@ inline int launch_pack(Plan* P, int pack, const PackLayout& Lo, int m, int n, in
     j.ns = Lo.ns;
     j.ks = Lo.ks;
     j.kr = kr;
-    j.tiles_m = tiles(Lo.ms, BM);
-    j.tiles_n = tiles(Lo.ns, BN);
+    j.tiles_m = tiles(Lo.ms, G.bm);
+    j.tiles_n = tiles(Lo.ns, G.bn);
     j.nprod = nprod;
     j.gsize = 1;
     j.ngroups = nprod;
     j.unit_base = JL.total_units;
     j.order_base = -1;
-    j.flags = vec_flags(a, ja, b, jb, even, M, Lo.ldm) | kJobStore;
+    j.flags = vec_flags(G, a, ja, b, jb, kb, M, Lo.ldm) | kJobStore;
     j.prods = prods;
     j.terms = terms;
     JL.total_units += nprod * j.tiles_m * j.tiles_n;
     JL.jobs.push_back(j);
   };
-  add_m_job(pa, la, pb, lb, b_even, 0, P->R, xp, xt);
-  if (Lo.kr > 0) add_m_job(A, lda, B, ldb, (Lo.ks & 1) == 0, Lo.kr, P->Mx * P->Nx, xp + P->R, xt);
-  rc = launch_jobs(JL, pack == kPackNaive ? 1 : P->max_nnz, nullptr, stream);
+  add_m_job(pa, la, pb, lb, kblk, 0, P->R, xp, xt);
+  if (Lo.kr > 0) add_m_job(A, lda, B, ldb, Lo.ks, Lo.kr, P->Mx * P->Nx, xp + P->R, xt);
+  rc = launch_jobs_dtype(dtype, JL, pack == kPackNaive ? 1 : P->max_nnz, nullptr, stream);
   if (rc != kOK) return rc;
-  AddJob ad{};
-  ad.M = M;
-  ad.C = C;
-  ad.ldm = Lo.ldm;
-  ad.ldc = ldc;
-  ad.rows = Lo.ms;
-  ad.cols = Lo.ns;
-  ad.mx = 1;
-  ad.nx = P->Nx;
-  ad.nlive = Lo.nlive;
-  ad.vec = ((uintptr_t)C & 15) == 0 && (ldc & 1) == 0 && ((uintptr_t)M & 15) == 0;
-  ad.cptr = cptr;
-  ad.cprod = cprod;
-  ad.cw = cw;
-  madd_kernel<<<dim3(ew_grid(Lo.ms, Lo.ns), P->Mx * P->Nx), dim3(kEwThreads), 0, stream>>>(ad);
+  const dim3 gm(ew_grid(Lo.ms, Lo.ns), P->Mx * P->Nx);
+  if (dtype == kF64) {
+    AddJob ad{};
+    ad.M = static_cast<const double*>(M);
+    ad.C = static_cast<double*>(C);
+    ad.ldm = Lo.ldm;
+    ad.ldc = ldc;
+    ad.rows = Lo.ms;
+    ad.cols = Lo.ns;
+    ad.mx = 1;
+    ad.nx = P->Nx;
+    ad.nlive = Lo.nlive;
+    ad.vec = ((uintptr_t)C & 15) == 0 && (ldc & 1) == 0 && ((uintptr_t)M & 15) == 0;
+    ad.cptr = cptr;
+    ad.cprod = cprod;
+    ad.cw = cw;
+    madd_kernel<<<gm, dim3(kEwThreads), 0, stream>>>(ad);
+  } else {
+    AddJobL ad{};
+    ad.M = static_cast<const float*>(M);
+    ad.C = static_cast<float*>(C);
+    ad.ldm = Lo.ldm;
+    ad.ldc = ldc;
+    ad.rows = Lo.ms;
+    ad.cols = Lo.ns;
+    ad.mx = 1;
+    ad.nx = P->Nx;
+    ad.nlive = Lo.nlive;
+    ad.vec = ((uintptr_t)C & 7) == 0 && (ldc & 1) == 0 && ((uintptr_t)M & 7) == 0;
+    ad.cptr = cptr;
+    ad.cprod = cprod;
+    ad.cw = cw;
+    madd_lowp_kernel<<<gm, dim3(kEwThreads), 0, stream>>>(ad);
+  }
   if (hipGetLastError() != hipSuccess) return kEHip;
   return kOK;
 }

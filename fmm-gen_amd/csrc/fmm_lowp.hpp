@@ -734,6 +734,131 @@ __global__ void __launch_bounds__(C1::NT) abc_lowp_kernel(Launch L) {
   });
 }
 
+// --- AB and Naive pack types with low-precision inputs --------------------
+//
+// Same structure as presum_kernel / madd_kernel (fmm_core.hpp): one leaf
+// block per grid.y, kEwRows x kEwCols tiles, two consecutive rows per lane.
+// The operand sums are formed in FP32 (s = u0*x0, s = fma(u_t, x_t, s)) and
+// stored rounded to the input type -- the rounding the ABC kernel applies when
+// it writes the sum to LDS -- so a product of the stored sums equals the ABC
+// product bit for bit; M and C are FP32.
+struct SumJobL {
+  const void* X;
+  void* Y;
+  long long ldx, ldy;
+  int rows, cols;
+  int ny;
+  int side;
+  int pad0, pad1;
+  const Prod* prods;
+  const Term* terms;
+};
+
+template <int IN>
+__device__ __forceinline__ void store_in(typename InT<IN>::T* p, float v) {
+  if constexpr (IN == kInF32) {
+    *(__attribute__((address_space(1))) float*)p = v;
+  } else if constexpr (IN == kInBF16) {
+    const __bf16 h = (__bf16)v;
+    *(__attribute__((address_space(1))) u16*)p = __builtin_bit_cast(u16, h);
+  } else {
+    const _Float16 h = (_Float16)v;
+    *(__attribute__((address_space(1))) u16*)p = __builtin_bit_cast(u16, h);
+  }
+}
+
+template <int IN>
+__global__ void __launch_bounds__(kEwThreads) presum_lowp_kernel(SumJobL S) {
+  typedef typename InT<IN>::T T;
+  const int r = (int)blockIdx.y;
+  int r0, c0;
+  ew_tile(S.rows, S.cols, &r0, &c0);
+  if (c0 >= S.cols) return;
+  const Prod* pp = S.prods + r;
+  const int t0 = S.side ? pp->b0 : pp->a0;
+  const int nt = S.side ? pp->nb : pp->na;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = r0 + 2 * lane;
+  T* y = static_cast<T*>(S.Y) + (long long)(r % S.ny) * S.rows + (long long)(r / S.ny) * S.cols * S.ldy;
+  for (int c = c0 + wave; c < min(c0 + kEwCols, S.cols); c += kEwThreads / 64) {
+    float s0 = 0.f, s1 = 0.f;
+    for (int t = 0; t < nt; ++t) {
+      const Term tm = S.terms[t0 + t];
+      const T* x = static_cast<const T*>(S.X) + (long long)tm.bi * S.rows + (long long)tm.bj * S.cols * S.ldx +
+                   (long long)c * S.ldx;
+      const float u = (float)tm.coef;
+      const float x0 = i < S.rows ? in_to_f<IN>(*(const __attribute__((address_space(1))) T*)(x + i)) : 0.f;
+      const float x1 = i + 1 < S.rows ? in_to_f<IN>(*(const __attribute__((address_space(1))) T*)(x + i + 1)) : 0.f;
+      s0 = t == 0 ? u * x0 : __builtin_fmaf(u, x0, s0);
+      s1 = t == 0 ? u * x1 : __builtin_fmaf(u, x1, s1);
+    }
+    T* yc = y + (long long)c * S.ldy;
+    if (i < S.rows) store_in<IN>(yc + i, s0);
+    if (i + 1 < S.rows) store_in<IN>(yc + i + 1, s1);
+  }
+}
+
+struct AddJobL {
+  const float* M;
+  float* C;
+  long long ldm, ldc;
+  int rows, cols;
+  int mx, nx;
+  int nlive;
+  int vec;
+  const int* cptr;
+  const int* cprod;
+  const double* cw;
+};
+
+__global__ void __launch_bounds__(kEwThreads) madd_lowp_kernel(AddJobL A) {
+  const int b = (int)blockIdx.y;
+  int r0, c0;
+  ew_tile(A.rows, A.cols, &r0, &c0);
+  if (c0 >= A.cols) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = r0 + 2 * lane;
+  const int l0 = A.cptr[b], l1 = A.cptr[b + 1];
+  gfloat* cb = (gfloat*)A.C + (long long)(b / A.nx) * A.rows + (long long)(b % A.nx) * A.cols * A.ldc;
+  for (int c = c0 + wave; c < min(c0 + kEwCols, A.cols); c += kEwThreads / 64) {
+    gfloat* cc = cb + (long long)c * A.ldc;
+    const bool v2 = A.vec && i + 1 < A.rows;
+    float c0v = 0.f, c1v = 0.f;
+    if (v2) {
+      const f32x2 v = *(gcf32x2*)(cc + i);
+      c0v = v.x;
+      c1v = v.y;
+    } else {
+      if (i < A.rows) c0v = cc[i];
+      if (i + 1 < A.rows) c1v = cc[i + 1];
+    }
+    for (int l = l0; l < l1; ++l) {
+      const int r = A.cprod[l];
+      if (r >= A.nlive) continue;
+      const float w = (float)A.cw[l];
+      gcfloat* m = (gcfloat*)A.M + (long long)(r % A.mx) * A.rows + (long long)(r / A.mx) * A.cols * A.ldm +
+                   (long long)c * A.ldm;
+      float m0 = 0.f, m1 = 0.f;
+      if (v2) {
+        const f32x2 v = *(gcf32x2*)(m + i);
+        m0 = v.x;
+        m1 = v.y;
+      } else {
+        if (i < A.rows) m0 = m[i];
+        if (i + 1 < A.rows) m1 = m[i + 1];
+      }
+      c0v = __builtin_fmaf(w, m0, c0v);
+      c1v = __builtin_fmaf(w, m1, c1v);
+    }
+    if (v2) {
+      *(__attribute__((address_space(1))) f32x2*)(cc + i) = f32x2{c0v, c1v};
+    } else {
+      if (i < A.rows) cc[i] = c0v;
+      if (i + 1 < A.rows) cc[i + 1] = c1v;
+    }
+  }
+}
+
 // The configurations built: 8 waves of 64x64 per workgroup, C tiles of
 // 256x128 (default) or 128x256 (jobs flagged kJobTileT), each with the
 // deepest slice and the largest prefetched term chunk that fit the 256 VGPRs

@@ -33,7 +33,7 @@ EXPORTS = ("fmm_plan_create", "fmm_plan_destroy", "fmm_plan_info", "fmm_workspac
            "fmm_ws_status", "fmm_host_abc", "fmm_host_run", "bl_dgemm", "bl_dgemm_ref",
            "bl_finalize", "fmm_batch_create", "fmm_batch_set", "fmm_batch_run", "fmm_batch_variant",
            "fmm_batch_info", "fmm_batch_status", "fmm_batch_destroy", "fmm_batch_reserve",
-           "fmm_batch_generation")
+           "fmm_batch_generation", "fmm_refine", "fmm_refine_workspace_bytes")
 
 
 class FmmError(RuntimeError):
@@ -382,3 +382,32 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+DTYPE_CODES = {v: k for k, v in DTYPE_NAMES.items()}
+
+
+def refine(A, B, C, main: str = "bf16", main_plan: Optional[Plan] = None, corr: str = "f32",
+           corr_plan: Optional[Plan] = None, stream=None, library=None) -> None:
+    """C += A @ B for FP64 device tensors: the main product A1 @ B1 in
+    `main` precision (FMM of main_plan, or plain GEMM), one correction
+    [dA | A1] @ [B1; dB] in `corr` precision ('f32' or 'bf16x3'), both added
+    to C in FP64 (see fmm_refine in include/fmm_amd.h)."""
+    import torch
+    L = library or lib()
+    if not getattr(L, "_refine_declared", False):
+        vp, i = ct.c_void_p, ct.c_int
+        L.fmm_refine.argtypes = [vp, i, vp, i, i, i, i, vp, i, vp, i, vp, i, vp, ct.c_size_t, vp]
+        L.fmm_refine.restype = i
+        L.fmm_refine_workspace_bytes.argtypes = [vp, i, vp, i, i, i, i]
+        L.fmm_refine_workspace_bytes.restype = ct.c_size_t
+        L._refine_declared = True
+    m, n, k = _check_operands(A, B, C)
+    for X in (A, B, C):
+        if X.dtype != torch.float64:
+            raise FmmError("refine takes float64 A, B, C")
+    mp = main_plan.handle if main_plan is not None else None
+    cp = corr_plan.handle if corr_plan is not None else None
+    check(L.fmm_refine(mp, DTYPE_CODES[main], cp, DTYPE_CODES[corr], m, n, k, A.data_ptr(), _ld(A),
+                       B.data_ptr(), _ld(B), C.data_ptr(), _ld(C), None, 0, _stream_ptr(stream)),
+          f"fmm_refine({main}, {corr})")

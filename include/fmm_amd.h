@@ -40,7 +40,8 @@ extern "C" {
  *   FMM_BF16X3  FP32 A, B; each operand sum split into BF16 hi + lo and every
  *               product formed as hi*hi + hi*lo + lo*hi on BF16 MFMA
  * The reference has FP64 only (common/include/bl_dgemm.h:52-130); the other
- * types have no counterpart there.  AB and NAIVE pack types are FP64 only. */
+ * types have no counterpart there.  Every pack type takes every dtype; the
+ * AB / NAIVE workspaces hold M in C's type and S, T in the input type. */
 #define FMM_F64     0
 #define FMM_F32     1
 #define FMM_BF16    2
@@ -79,7 +80,8 @@ int fmm_plan_info(fmm_plan_t plan, int *Mx, int *Kx, int *Nx, int *rank);
  * FMM_F64 and is enough for every dtype (the others use larger C tiles). */
 size_t fmm_workspace_bytes(fmm_plan_t plan, int m, int n, int k);
 /* The same for any pack type: AB needs M (R [+ Mx*Nx] leaf blocks ms x ns),
- * NAIVE also S (R blocks ms x ks) and T (R blocks ks x ns). */
+ * NAIVE also S (R blocks ms x ks) and T (R blocks ks x ns).  Sized for
+ * FMM_F64 elements, which bounds every other dtype. */
 size_t fmm_workspace_bytes_pack(fmm_plan_t plan, int pack, int m, int n, int k);
 
 /* ---- device-pointer entries (stream ordered, reentrant) -------------------
@@ -158,6 +160,26 @@ int fmm_batch_generation(fmm_batch_t batch);
 int fmm_batch_info(fmm_batch_t batch, int *jobs, int *units);
 int fmm_batch_status(fmm_batch_t batch, int *flags);
 int fmm_batch_destroy(fmm_batch_t batch);
+
+/* ---- FP64 product through low-precision products and one correction -------
+ * No counterpart in the reference.  C += A*B for FP64 A, B, C (device
+ * pointers): with A1 = fl_T(A), B1 = fl_T(B) (T = the input type of
+ * main_dtype: FMM_F32, FMM_BF16, FMM_F16 or FMM_BF16X3) and the FP32
+ * remainders dA = fl32(A - A1), dB = fl32(B - B1),
+ *     C~ = A1*B1              by main_plan (NULL: plain GEMM), FP32 result;
+ *     E  = [dA | A1]*[B1; dB] by corr_plan (NULL: plain GEMM) of corr_dtype
+ *                             (FMM_F32 or FMM_BF16X3), depth 2k, FP32 result;
+ *     C += C~ + E             in FP64.
+ * E is the residual A*B - C~ up to dA*dB and C~'s own arithmetic error, so
+ * the correction removes the input-rounding error of the main product.  ws
+ * of fmm_refine_workspace_bytes bytes, or NULL (stream-ordered allocation). */
+size_t fmm_refine_workspace_bytes(fmm_plan_t main_plan, int main_dtype,
+                                  fmm_plan_t corr_plan, int corr_dtype,
+                                  int m, int n, int k);
+int fmm_refine(fmm_plan_t main_plan, int main_dtype, fmm_plan_t corr_plan,
+               int corr_dtype, int m, int n, int k, const double *A, int lda,
+               const double *B, int ldb, double *C, int ldc, void *ws,
+               size_t ws_bytes, void *stream);
 
 /* ---- host-pointer drop-ins -------------------------------------------------
  * Copy in, run on the current GPU, copy C back; return when done.

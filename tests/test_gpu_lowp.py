@@ -159,5 +159,37 @@ def test_lowp_argument_errors(torch, fmm, rt):
         plan.abc(A, rt.colmajor(64, 64, dtype=torch.float16), C32)   # mixed inputs
     with pytest.raises(rt.FmmError):
         plan.abc(A, B, C32, split=True)                      # split needs FP32 inputs
-    with pytest.raises(rt.FmmError, match="FMM_EUNSUPPORTED"):
-        plan.run(A, B, C32, pack="ab")                       # AB / Naive are FP64 only
+
+
+@pytest.mark.parametrize("pack", ["ab", "naive"])
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("spec", [(["222"], [1]), (["333"], [1]), (["222", "323"], [1, 1])],
+                         ids=["222L1", "333L1", "222x323"])
+def test_lowp_packs_equal_abc(torch, fmm, rt, oracle, spec, dtype, pack):
+    """AB and Naive with low-precision inputs: the stored operand sums are
+    rounded like the ABC kernel's LDS sums and M_Add applies the same FP32
+    updates in the same order, so the results equal ABC's bit for bit."""
+    alg = fmm.build_algorithm(*spec)
+    plan = rt.Plan(alg)
+    Mx, Kx, Nx = alg.dims
+    rng = np.random.default_rng(31)
+    ti = in_type(torch, dtype)
+    split = dtype == "bf16x3"
+    for (m, n, k) in [(Mx * 8 * 33 + 7, Nx * 4 * 33 + 3, Kx * 37 + 1), (Mx * 8 * 40, Nx * 4 * 40, Kx * 70)]:
+        A = stored(oracle, rng.uniform(-1, 1, (m, k)), dtype)
+        B = stored(oracle, rng.uniform(-1, 1, (k, n)), dtype)
+        C = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+        outs = {}
+        for pk in ("abc", pack):
+            dA = to_dev(torch, rt, A, ti)
+            dB = to_dev(torch, rt, B, ti)
+            dC = to_dev(torch, rt, C, torch.float32)
+            plan.run(dA, dB, dC, pack=pk, split=split)
+            torch.cuda.synchronize()
+            outs[pk] = dC.cpu().numpy()
+        assert np.array_equal(outs["abc"], outs[pack]), (spec, dtype, pack, m, n, k)
+        Ai, Bi, Ci = int_mats(rng, m, n, k)
+        dA, dB, dC = to_dev(torch, rt, Ai, ti), to_dev(torch, rt, Bi, ti), to_dev(torch, rt, Ci, torch.float32)
+        plan.run(dA, dB, dC, pack=pack, split=split)
+        torch.cuda.synchronize()
+        assert np.array_equal(dC.cpu().numpy(), Ci + Ai @ Bi), (spec, dtype, pack, "int", m, n, k)

@@ -109,7 +109,8 @@ def test_pack_workspace(torch, fmm, rt, pack):
     Mblocks = alg.rank + 9            # products + k-fringe products
     want = -(-8 * ms * ns * Mblocks // 256) * 256
     if pack == "naive":
-        want += -(-8 * ms * ks * alg.rank // 256) * 256 + -(-8 * ks * ns * alg.rank // 256) * 256
+        ldt = -(-ks // 8) * 8             # T's leading dimension: a multiple of 8 elements
+        want += -(-8 * ms * ks * alg.rank // 256) * 256 + -(-8 * ldt * ns * alg.rank // 256) * 256
     assert nbytes == want
     rng = np.random.default_rng(18)
     A, B, C = int_mats(rng, m, n, k)
