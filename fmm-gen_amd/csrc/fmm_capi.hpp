@@ -51,7 +51,7 @@ inline int run_abc(Plan* P, int dtype, int m, int n, int k, const void* A, int l
   } else if (need > ws_bytes && w) {
     return kEInval;
   }
-  rc = launch_jobs_dtype(dtype, JL, P->max_nnz, w, s);
+  rc = launch_jobs_dtype(dtype, JL, P->max_nnz, w, s, P->unit_single);
   if (own) (void)hipFreeAsync(w, s);
   return rc;
 }
@@ -103,7 +103,7 @@ inline int run_gemm(int dtype, int m, int n, int k, const void* A, int lda, cons
   if (rc != kOK) return rc;
   JobList JL;
   add_gemm_job(JL, G, ip, it, m, n, k, A, lda, B, ldb, C, ldc);
-  return launch_jobs_dtype(dtype, JL, 1, nullptr, s);
+  return launch_jobs_dtype(dtype, JL, 1, nullptr, s, true);
 }
 
 // Staging buffers of the host-pointer drop-ins (grow-only, one set per

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -83,6 +84,7 @@ struct Plan {
   std::vector<int> dims;   // 3*nlev
   int Mx = 1, Kx = 1, Nx = 1, R = 0;
   int max_nnz = 0;         // largest per-product source count (A or B)
+  bool unit_single = true; // every product: one A and one B term, coefficient +-1
   // ABC: R products, then Mx*Nx k-fringe products.
   Table main;
   // Naive: product r reads workspace blocks S_r, T_r and writes M_r.
@@ -138,6 +140,7 @@ inline int build_plan(Plan* P, int nlev, const int* dims, int R, const double* U
   prods.clear();
   terms.clear();
   P->max_nnz = 0;
+  P->unit_single = true;
   const int rowsU = P->Mx * P->Kx, rowsV = P->Kx * P->Nx, rowsW = P->Mx * P->Nx;
   for (int r = 0; r < R; ++r) {
     Prod pr{};
@@ -158,6 +161,8 @@ inline int build_plan(Plan* P, int nlev, const int* dims, int R, const double* U
     collect(W, rowsW, 2, &pr.c0, &pr.nc);
     pr.fringe = 0;
     P->max_nnz = std::max(P->max_nnz, std::max(pr.na, pr.nb));
+    if (!(pr.na == 1 && pr.nb == 1 && std::fabs(terms[pr.a0].coef) == 1.0 && std::fabs(terms[pr.b0].coef) == 1.0))
+      P->unit_single = false;
     prods.push_back(pr);
   }
   // k-fringe: C(I,J) += A(I, Kx) B(Kx, J) over depth kr, as one-term products
@@ -631,12 +636,25 @@ inline int launch_lowp_shapes(const JobList& JL, int* ws, hipStream_t stream) {
   return launch_lowp<C1, C1>(JL, ws, stream);
 }
 
-inline int launch_jobs_dtype(int dtype, const JobList& JL, int max_terms, int* ws, hipStream_t stream) {
+// `one`: every product of the launch has single +-1 operand terms (plain
+// GEMMs, Naive products): the low-precision types then run their
+// single-term kernels (deeper slices, no operand-sum code).
+inline int launch_jobs_dtype(int dtype, const JobList& JL, int max_terms, int* ws, hipStream_t stream,
+                             bool one = false) {
+  bool t = false;
+  for (const Job& j : JL.jobs) t = t || (j.flags & kJobTileT) != 0;
+  const bool use_one = one && !t;
   switch (dtype) {
     case kF64: return launch_jobs(JL, max_terms, ws, stream);
-    case kF32: return launch_lowp_shapes<CfgF32, CfgF32T>(JL, ws, stream);
-    case kBF16: return launch_lowp_shapes<CfgBF16, CfgBF16T>(JL, ws, stream);
-    case kF16: return launch_lowp_shapes<CfgF16, CfgF16T>(JL, ws, stream);
+    case kF32:
+      return use_one ? launch_lowp<CfgF32One, CfgF32One>(JL, ws, stream)
+                     : launch_lowp_shapes<CfgF32, CfgF32T>(JL, ws, stream);
+    case kBF16:
+      return use_one ? launch_lowp<CfgBF16One, CfgBF16One>(JL, ws, stream)
+                     : launch_lowp_shapes<CfgBF16, CfgBF16T>(JL, ws, stream);
+    case kF16:
+      return use_one ? launch_lowp<CfgF16One, CfgF16One>(JL, ws, stream)
+                     : launch_lowp_shapes<CfgF16, CfgF16T>(JL, ws, stream);
     case kBF16x3: return launch_lowp<CfgBF16x3, CfgBF16x3>(JL, ws, stream);
     default: return kEUnsupported;
   }
@@ -822,7 +840,7 @@ inline int launch_pack(Plan* P, int pack, int dtype, const PackLayout& Lo, int m
   };
   add_m_job(pa, la, pb, lb, kblk, 0, P->R, xp, xt);
   if (Lo.kr > 0) add_m_job(A, lda, B, ldb, Lo.ks, Lo.kr, P->Mx * P->Nx, xp + P->R, xt);
-  rc = launch_jobs_dtype(dtype, JL, pack == kPackNaive ? 1 : P->max_nnz, nullptr, stream);
+  rc = launch_jobs_dtype(dtype, JL, pack == kPackNaive ? 1 : P->max_nnz, nullptr, stream, pack == kPackNaive);
   if (rc != kOK) return rc;
   const dim3 gm(ew_grid(Lo.ms, Lo.ns), P->Mx * P->Nx);
   if (dtype == kF64) {

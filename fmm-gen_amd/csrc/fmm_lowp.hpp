@@ -92,13 +92,18 @@ __device__ __forceinline__ void unpack2(unsigned int w, float& lo, float& hi) {
 }
 
 // Kernel configuration.
-template <int IN_, int OP_, bool SPLIT_, int BM_, int BN_, int BK_, int WM_, int WN_, int CH_>
+template <int IN_, int OP_, bool SPLIT_, int BM_, int BN_, int BK_, int WM_, int WN_, int CH_, int PF_ = 1,
+          bool ONE_ = false>
 struct LCfg {
   static constexpr int IN = IN_, OP = OP_;
   static constexpr bool SPLIT = SPLIT_;
   static constexpr int BM = BM_, BN = BN_, BK = BK_;
   static constexpr int WM = WM_, WN = WN_;            // waves along m and n
   static constexpr int CH = CH_;                      // operand terms per prefetched chunk
+  static constexpr int PF = PF_;                      // slices of loads in flight (register ring depth)
+  // Every product of the launch has one term per operand with coefficient
+  // +-1 (plain GEMMs, Naive products): no operand-sum code at all.
+  static constexpr bool ONE = ONE_;
   static constexpr int NT = 64 * WM * WN;
   static constexpr int WTM = BM / WM, WTN = BN / WN;  // wave tile
   static constexpr int FM = WTM / 16, FN = WTN / 16;  // MFMA blocks per wave
@@ -131,6 +136,8 @@ struct LCfg {
   static_assert(IN != kInF32 || !H16 || AR == 2 || AR == 4, "FP32 A item");
   static_assert(LDS_BYTES <= 160 * 1024, "LDS");
   static_assert(!H16 || NKG == 4 || NKG == 8, "16-bit slice depth 32 or 64");
+  static_assert(PF == 1 || PF == 2, "prefetch depth");
+  static_assert(!ONE || RAWCOPY, "single-term variant copies raw operands");
 };
 
 // Physical k group of logical k group `kg` in 16-bit LDS row (or column) r.
@@ -571,48 +578,70 @@ __device__ __forceinline__ void l_mfma(const char* As, const char* Bs, int lane,
   __builtin_amdgcn_s_setprio(0);
 }
 
+// One slice of the pipelined contraction: MFMAs on LDS buffer `cur` while
+// the sums of slice st+1 (its loads were issued PF slices earlier into `r`)
+// go to the other buffer and the loads of slice st+1+PF are issued into `r`.
+// The two halves of the workgroup's waves run the phases in opposite order,
+// so one half's packing overlaps the other half's MFMAs on the same SIMD.
 template <class C>
-__device__ __forceinline__ void l_sums_phase(const Job& J, const Prod& P, const LSrcs<C>& S, int st, int nsteps,
-                                             const LGeo<C>& g, LRaw<C>& r, char* As, char* Bs, bool fa, bool fb) {
-  if (st + 1 < nsteps) {
-    l_pack<C>(J, P, S, (st + 1) * C::BK, g, r, As, Bs, fa, fb);
-    if (st + 2 < nsteps) l_load_raw<C>(J, S, (st + 2) * C::BK, g, r);
+__device__ __forceinline__ void l_step(const Job& J, const Prod& P, const LSrcs<C>& S, int st, int nsteps,
+                                       const LGeo<C>& g, LRaw<C>& r, char* lds, int lane, int wm, int wn, int grp,
+                                       bool fa, bool fb, f32x4 (&acc)[C::FM][C::FN]) {
+  constexpr int KS = C::BK / C::KM;   // MFMA k-steps per slice
+  constexpr int BUF = C::NPL * (C::A_ELEMS + C::B_ELEMS) * C::OPSZ;
+  constexpr int BOFF = C::NPL * C::A_ELEMS * C::OPSZ;
+  char* cur = lds + (st & 1) * BUF;
+  char* nxt = lds + ((st & 1) ^ 1) * BUF;
+  auto sums = [&]() {
+    if (st + 1 < nsteps) {
+      l_pack<C>(J, P, S, (st + 1) * C::BK, g, r, nxt, nxt + BOFF, fa, fb);
+      if (st + 1 + C::PF < nsteps) l_load_raw<C>(J, S, (st + 1 + C::PF) * C::BK, g, r);
+    }
+  };
+  if (grp == 0) {
+    l_mfma<C, 0, KS / 2>(cur, cur + BOFF, lane, wm, wn, acc);
+    sums();
+    l_mfma<C, KS / 2, KS>(cur, cur + BOFF, lane, wm, wn, acc);
+  } else {
+    sums();
+    l_mfma<C, 0, KS>(cur, cur + BOFF, lane, wm, wn, acc);
   }
+  __syncthreads();
 }
 
-// Contraction of one unit: double-buffered LDS slices, one barrier per slice;
-// the loads of slice st+2 are in flight while slice st is multiplied.  The two
-// halves of the workgroup's waves run the phases in opposite order, so one
-// half's packing overlaps the other half's MFMAs on the same SIMD.
+// Contraction of one unit: double-buffered LDS slices, one barrier per slice,
+// PF register buffers of raw loads in flight (slice s uses buffer s % PF
+// after the first; the loop is unrolled by PF so the buffer choice is
+// static).
 template <class C>
 __device__ __forceinline__ void l_contract(const Job& J, const Prod& P, const LGeo<C>& g, char* lds, int lane, int wm,
                                            int wn, int grp, f32x4 (&acc)[C::FM][C::FN]) {
-  constexpr int KS = C::BK / C::KM;   // MFMA k-steps per slice
-  constexpr int BUF = C::NPL * (C::A_ELEMS + C::B_ELEMS) * C::OPSZ;
   constexpr int BOFF = C::NPL * C::A_ELEMS * C::OPSZ;
   const int nsteps = (g.depth + C::BK - 1) / C::BK;
   if (!(P.na > 0 && P.nb > 0 && nsteps > 0)) return;
   LSrcs<C> S;
   l_make_srcs<C>(J, P.a0, P.na, P.b0, P.nb, S);
-  const bool fa = C::RAWCOPY && P.na == 1 && __builtin_fabsf(S.u[0]) == 1.f;
-  const bool fb = C::RAWCOPY && P.nb == 1 && __builtin_fabsf(S.v[0]) == 1.f;
-  LRaw<C> r;
-  l_load_raw<C>(J, S, 0, g, r);
-  l_pack<C>(J, P, S, 0, g, r, lds, lds + BOFF, fa, fb);
-  if (nsteps > 1) l_load_raw<C>(J, S, C::BK, g, r);
-  __syncthreads();
-  for (int st = 0; st < nsteps; ++st) {
-    char* cur = lds + (st & 1) * BUF;
-    char* nxt = lds + ((st & 1) ^ 1) * BUF;
-    if (grp == 0) {
-      l_mfma<C, 0, KS / 2>(cur, cur + BOFF, lane, wm, wn, acc);
-      l_sums_phase<C>(J, P, S, st, nsteps, g, r, nxt, nxt + BOFF, fa, fb);
-      l_mfma<C, KS / 2, KS>(cur, cur + BOFF, lane, wm, wn, acc);
-    } else {
-      l_sums_phase<C>(J, P, S, st, nsteps, g, r, nxt, nxt + BOFF, fa, fb);
-      l_mfma<C, 0, KS>(cur, cur + BOFF, lane, wm, wn, acc);
-    }
+  const bool fa = C::ONE || (C::RAWCOPY && P.na == 1 && __builtin_fabsf(S.u[0]) == 1.f);
+  const bool fb = C::ONE || (C::RAWCOPY && P.nb == 1 && __builtin_fabsf(S.v[0]) == 1.f);
+  if constexpr (C::PF == 1) {
+    LRaw<C> r;
+    l_load_raw<C>(J, S, 0, g, r);
+    l_pack<C>(J, P, S, 0, g, r, lds, lds + BOFF, fa, fb);
+    if (nsteps > 1) l_load_raw<C>(J, S, C::BK, g, r);
     __syncthreads();
+    for (int st = 0; st < nsteps; ++st) l_step<C>(J, P, S, st, nsteps, g, r, lds, lane, wm, wn, grp, fa, fb, acc);
+  } else {
+    // slice 0 via ra; then odd slices in ra, even slices in rb
+    LRaw<C> ra, rb;
+    l_load_raw<C>(J, S, 0, g, ra);
+    l_pack<C>(J, P, S, 0, g, ra, lds, lds + BOFF, fa, fb);
+    if (nsteps > 1) l_load_raw<C>(J, S, C::BK, g, ra);
+    if (nsteps > 2) l_load_raw<C>(J, S, 2 * C::BK, g, rb);
+    __syncthreads();
+    for (int st = 0; st < nsteps; st += 2) {
+      l_step<C>(J, P, S, st, nsteps, g, ra, lds, lane, wm, wn, grp, fa, fb, acc);
+      if (st + 1 < nsteps) l_step<C>(J, P, S, st + 1, nsteps, g, rb, lds, lane, wm, wn, grp, fa, fb, acc);
+    }
   }
 }
 
@@ -860,17 +889,51 @@ __global__ void __launch_bounds__(kEwThreads) madd_lowp_kernel(AddJobL A) {
 }
 
 // The configurations built: 8 waves of 64x64 per workgroup, C tiles of
-// 256x128 (default) or 128x256 (jobs flagged kJobTileT), each with the
-// deepest slice and the largest prefetched term chunk that fit the 256 VGPRs
-// of two waves per SIMD without spilling (operand sums with more terms load
-// their remaining chunks synchronously).  Split-BF16 has the 256x128 shape
-// only (its hi and lo planes leave no LDS for the deeper 128x256 slices).
-typedef LCfg<kInF32, kOpF32, false, 256, 128, 16, 4, 2, 2> CfgF32;
-typedef LCfg<kInF32, kOpF32, false, 128, 256, 16, 2, 4, 2> CfgF32T;
-typedef LCfg<kInBF16, kOpBF16, false, 256, 128, 32, 4, 2, 2> CfgBF16;
-typedef LCfg<kInBF16, kOpBF16, false, 128, 256, 32, 2, 4, 1> CfgBF16T;
-typedef LCfg<kInF16, kOpF16, false, 256, 128, 32, 4, 2, 2> CfgF16;
-typedef LCfg<kInF16, kOpF16, false, 128, 256, 32, 2, 4, 1> CfgF16T;
-typedef LCfg<kInF32, kOpBF16, true, 256, 128, 32, 4, 2, 1> CfgBF16x3;
+// 256x128 (default) or 128x256 (jobs flagged kJobTileT).  Slice depth,
+// prefetched term chunk and prefetch depth were chosen by timing 8192^3
+// GEMMs and FMMs on the device among the variants that fit the 256 VGPRs of
+// two waves per SIMD (a few spilled registers where that measured faster):
+// deeper slices and two slices of loads in flight hide the L2 / HBM latency
+// that a 32-deep BF16 slice (about 500 MFMA cycles per SIMD) cannot.  Operand
+// sums with more terms than a chunk load the rest synchronously.  Split-BF16
+// has the 256x128 shape only (its hi and lo planes leave no LDS for the
+// deeper 128x256 slices).
+// (slice depth BK, term chunk CH, prefetch depth PF) per configuration
+#ifndef FMM_F32_CFG
+#define FMM_F32_CFG 16, 2, 2
+#endif
+#ifndef FMM_F32T_CFG
+#define FMM_F32T_CFG 16, 2, 1
+#endif
+#ifndef FMM_H16_CFG
+#define FMM_H16_CFG 32, 2, 2
+#endif
+#ifndef FMM_H16T_CFG
+#define FMM_H16T_CFG 32, 1, 1
+#endif
+#ifndef FMM_SPLIT_CFG
+#define FMM_SPLIT_CFG 32, 1, 1
+#endif
+#ifndef FMM_F32_ONE_CFG
+#define FMM_F32_ONE_CFG 32, 1, 2
+#endif
+#ifndef FMM_H16_ONE_CFG
+#define FMM_H16_ONE_CFG 64, 1, 2
+#endif
+template <int IN, int OP, bool SPLIT, int BM, int BN, int WM, int WN, int BK, int CH, int PF>
+using LC = LCfg<IN, OP, SPLIT, BM, BN, BK, WM, WN, CH, PF>;
+template <int IN, int OP, int BM, int BN, int WM, int WN, int BK, int CH, int PF>
+using LC1 = LCfg<IN, OP, false, BM, BN, BK, WM, WN, CH, PF, true>;
+typedef LC<kInF32, kOpF32, false, 256, 128, 4, 2, FMM_F32_CFG> CfgF32;
+typedef LC<kInF32, kOpF32, false, 128, 256, 2, 4, FMM_F32T_CFG> CfgF32T;
+typedef LC<kInBF16, kOpBF16, false, 256, 128, 4, 2, FMM_H16_CFG> CfgBF16;
+typedef LC<kInBF16, kOpBF16, false, 128, 256, 2, 4, FMM_H16T_CFG> CfgBF16T;
+typedef LC<kInF16, kOpF16, false, 256, 128, 4, 2, FMM_H16_CFG> CfgF16;
+typedef LC<kInF16, kOpF16, false, 128, 256, 2, 4, FMM_H16T_CFG> CfgF16T;
+typedef LC<kInF32, kOpBF16, true, 256, 128, 4, 2, FMM_SPLIT_CFG> CfgBF16x3;
+// single-term (+-1) launches: plain GEMMs and Naive product launches
+typedef LC1<kInF32, kOpF32, 256, 128, 4, 2, FMM_F32_ONE_CFG> CfgF32One;
+typedef LC1<kInBF16, kOpBF16, 256, 128, 4, 2, FMM_H16_ONE_CFG> CfgBF16One;
+typedef LC1<kInF16, kOpF16, 256, 128, 4, 2, FMM_H16_ONE_CFG> CfgF16One;
 
 }  // namespace fmm
