@@ -181,7 +181,8 @@ def ev_ms(fn, reps=3):
 
 
 def section_precision(F, rt, seed):
-    """Config 'precision sweep': 8192^3, FP64 U[-1,1) inputs cast per dtype."""
+    """Config 'precision sweep': 8192^3, FP64 U[-1,1) inputs cast per dtype;
+    each FMM in the ABC and Naive pack types."""
     import torch
     N = 8192
     gen = torch.Generator(device="cuda").manual_seed(seed)
@@ -201,29 +202,35 @@ def section_precision(F, rt, seed):
         B = rt.as_colmajor(B64.to(ti))
         split = dt == "bf16x3"
         for a, plan in plans.items():
-            def call():
-                if plan is None:
-                    rt.gemm(A, B, C, split=split)
-                else:
-                    plan.abc(A, B, C, split=split)
-            C.zero_()
-            call()
-            torch.cuda.synchronize()
-            err = rel_sampled(A64, B64, Z64, C)
-            ms = ev_ms(call)
-            rows.append({"dtype": dt, "alg": a, "ms": round(ms, 3), "eff_tflops": round(fl / ms / 1e9, 1),
-                         "rel_err": float(f"{err:.3e}")})
+            for pack in (("gemm",) if plan is None else ("abc", "naive")):
+                ws = None if plan is None else torch.empty(plan.workspace_bytes(N, N, N, pack) + 256,
+                                                           dtype=torch.uint8, device="cuda")
+
+                def call():
+                    if plan is None:
+                        rt.gemm(A, B, C, split=split)
+                    else:
+                        plan.run(A, B, C, pack=pack, ws=ws, split=split)
+                C.zero_()
+                call()
+                torch.cuda.synchronize()
+                err = rel_sampled(A64, B64, Z64, C)
+                ms = ev_ms(call)
+                rows.append({"dtype": dt, "alg": a, "pack": pack, "ms": round(ms, 3),
+                             "eff_tflops": round(fl / ms / 1e9, 1), "rel_err": float(f"{err:.3e}")})
+                del ws
         del A, B
     C64 = rt.colmajor(N, N)
-    for main, a in (("bf16", "gemm"), ("bf16", "222L1"), ("f16", "gemm"), ("f32", "222L1"), ("bf16x3", "222L1")):
+    for main, a, corr in (("bf16", "gemm", "f32"), ("bf16", "gemm", "bf16"), ("bf16", "222L1", "bf16"),
+                          ("f16", "gemm", "f32"), ("f32", "222L1", "f32"), ("bf16x3", "222L1", "bf16x3")):
         def call():
-            rt.refine(A64, B64, C64, main=main, main_plan=plans[a], corr="f32")
+            rt.refine(A64, B64, C64, main=main, main_plan=plans[a], corr=corr)
         C64.zero_()
         call()
         torch.cuda.synchronize()
         err = rel_sampled(A64, B64, Z64, C64)
         ms = ev_ms(call, 2)
-        rows.append({"dtype": f"{main}+refine(f32)", "alg": a, "ms": round(ms, 3),
+        rows.append({"dtype": f"{main}+refine({corr})", "alg": a, "pack": "abc" if plans[a] else "gemm", "ms": round(ms, 3),
                      "eff_tflops": round(fl / ms / 1e9, 1), "rel_err": float(f"{err:.3e}")})
     # best FMM vs the same dtype's plain GEMM (both within that dtype's stated
     # accuracy, tests/tolerance.py)
@@ -231,7 +238,7 @@ def section_precision(F, rt, seed):
     for dt in ("f32", "bf16", "f16", "bf16x3"):
         g = next(r for r in rows if r["dtype"] == dt and r["alg"] == "gemm")
         f = max((r for r in rows if r["dtype"] == dt and r["alg"] != "gemm"), key=lambda r: r["eff_tflops"])
-        best[dt] = {"best_fmm": f["alg"], "fmm_tflops": f["eff_tflops"], "gemm_tflops": g["eff_tflops"],
+        best[dt] = {"best_fmm": f"{f['alg']}/{f['pack']}", "fmm_tflops": f["eff_tflops"], "gemm_tflops": g["eff_tflops"],
                     "speedup": round(f["eff_tflops"] / g["eff_tflops"], 3)}
     return {"shape": [N, N, N], "inputs": "FP64 U[-1,1) cast per dtype, C = 0",
             "err_metric": "relative Frobenius error vs the FP64 product of the FP64 inputs, 512 sampled entries",

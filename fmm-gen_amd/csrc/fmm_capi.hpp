@@ -386,15 +386,14 @@ int fmm_batch_destroy(fmm_batch_t batch) {
 namespace fmm {
 inline bool refine_types_ok(int main_dtype, int corr_dtype) {
   return (main_dtype == kF32 || main_dtype == kBF16 || main_dtype == kF16 || main_dtype == kBF16x3) &&
-         (corr_dtype == kF32 || corr_dtype == kBF16x3);
+         (corr_dtype == kF32 || corr_dtype == kBF16x3 || (corr_dtype == kBF16 && main_dtype == kBF16));
 }
 
 inline size_t refine_ws(fmm_plan_t mp, int main_dtype, fmm_plan_t cp, int corr_dtype, int m, int n, int k,
                         RefLayout* out) {
   const size_t wm = mp ? workspace_bytes(&mp->p, kPackABC, m, n, k) : 0;
   const size_t wc = cp ? workspace_bytes(&cp->p, kPackABC, m, n, 2 * k) : 0;
-  *out = ref_layout(main_dtype, m, n, k, wm, wc);
-  (void)corr_dtype;
+  *out = ref_layout(main_dtype, corr_dtype, m, n, k, wm, wc);
   return out->bytes;
 }
 }  // namespace fmm
@@ -426,8 +425,8 @@ int fmm_refine(fmm_plan_t main_plan, int main_dtype, fmm_plan_t corr_plan, int c
   } else if (ws_bytes < need) {
     return kEInval;
   }
-  float* AA = reinterpret_cast<float*>(w + Lo.aa);
-  float* BB = reinterpret_cast<float*>(w + Lo.bb);
+  void* AA = w + Lo.aa;
+  void* BB = w + Lo.bb;
   float* Ct = reinterpret_cast<float*>(w + Lo.ct);
   float* Et = reinterpret_cast<float*>(w + Lo.et);
   const long long mk = (long long)m * k, kn = (long long)k * n, mn = (long long)Lo.lt * n;
@@ -435,25 +434,27 @@ int fmm_refine(fmm_plan_t main_plan, int main_dtype, fmm_plan_t corr_plan, int c
   auto chk = [&]() {
     if (rc == kOK && hipGetLastError() != hipSuccess) rc = kEHip;
   };
-  switch (main_dtype) {
-    case kBF16:
-      split_a_kernel<kInBF16><<<ew_blocks(mk), kRefThreads, 0, s>>>(m, k, A, lda, reinterpret_cast<u16*>(w + Lo.a1),
-                                                                    Lo.la1, AA, Lo.laa);
-      split_b_kernel<kInBF16><<<ew_blocks(kn), kRefThreads, 0, s>>>(k, n, B, ldb, reinterpret_cast<u16*>(w + Lo.b1),
-                                                                    Lo.lb1, BB, Lo.lbb);
-      break;
-    case kF16:
-      split_a_kernel<kInF16><<<ew_blocks(mk), kRefThreads, 0, s>>>(m, k, A, lda, reinterpret_cast<u16*>(w + Lo.a1),
-                                                                   Lo.la1, AA, Lo.laa);
-      split_b_kernel<kInF16><<<ew_blocks(kn), kRefThreads, 0, s>>>(k, n, B, ldb, reinterpret_cast<u16*>(w + Lo.b1),
-                                                                   Lo.lb1, BB, Lo.lbb);
-      break;
-    default:   // FP32 inputs (FMM_F32, FMM_BF16X3)
-      split_a_kernel<kInF32><<<ew_blocks(mk), kRefThreads, 0, s>>>(m, k, A, lda, reinterpret_cast<float*>(w + Lo.a1),
-                                                                   Lo.la1, AA, Lo.laa);
-      split_b_kernel<kInF32><<<ew_blocks(kn), kRefThreads, 0, s>>>(k, n, B, ldb, reinterpret_cast<float*>(w + Lo.b1),
-                                                                   Lo.lb1, BB, Lo.lbb);
-      break;
+  u16* a1h = reinterpret_cast<u16*>(w + Lo.a1);
+  u16* b1h = reinterpret_cast<u16*>(w + Lo.b1);
+  float* a1f = reinterpret_cast<float*>(w + Lo.a1);
+  float* b1f = reinterpret_cast<float*>(w + Lo.b1);
+  float* aaf = static_cast<float*>(AA);
+  float* bbf = static_cast<float*>(BB);
+  u16* aah = static_cast<u16*>(AA);
+  u16* bbh = static_cast<u16*>(BB);
+  const int ga = ew_blocks(mk), gb = ew_blocks(kn);
+  if (main_dtype == kBF16 && corr_dtype == kBF16) {
+    split_a_kernel<kInBF16, kInBF16><<<ga, kRefThreads, 0, s>>>(m, k, A, lda, a1h, Lo.la1, aah, Lo.laa);
+    split_b_kernel<kInBF16, kInBF16><<<gb, kRefThreads, 0, s>>>(k, n, B, ldb, b1h, Lo.lb1, bbh, Lo.lbb);
+  } else if (main_dtype == kBF16) {
+    split_a_kernel<kInBF16, kInF32><<<ga, kRefThreads, 0, s>>>(m, k, A, lda, a1h, Lo.la1, aaf, Lo.laa);
+    split_b_kernel<kInBF16, kInF32><<<gb, kRefThreads, 0, s>>>(k, n, B, ldb, b1h, Lo.lb1, bbf, Lo.lbb);
+  } else if (main_dtype == kF16) {
+    split_a_kernel<kInF16, kInF32><<<ga, kRefThreads, 0, s>>>(m, k, A, lda, a1h, Lo.la1, aaf, Lo.laa);
+    split_b_kernel<kInF16, kInF32><<<gb, kRefThreads, 0, s>>>(k, n, B, ldb, b1h, Lo.lb1, bbf, Lo.lbb);
+  } else {   // FP32 inputs (FMM_F32, FMM_BF16X3)
+    split_a_kernel<kInF32, kInF32><<<ga, kRefThreads, 0, s>>>(m, k, A, lda, a1f, Lo.la1, aaf, Lo.laa);
+    split_b_kernel<kInF32, kInF32><<<gb, kRefThreads, 0, s>>>(k, n, B, ldb, b1f, Lo.lb1, bbf, Lo.lbb);
   }
   chk();
   zero_f32_kernel<<<ew_blocks(mn), kRefThreads, 0, s>>>(Ct, mn);

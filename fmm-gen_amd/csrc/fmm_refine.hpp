@@ -7,8 +7,10 @@
 // The main product C~ = A1*B1 runs on the fast path (an FMM or GEMM of type
 // T, FP32 result); the correction E = dA*B1 + A1*dB -- the residual A*B - C~
 // up to the dropped dA*dB and the main product's own arithmetic error -- is
-// one product of FP32 inputs with depth 2k, [dA | A1] * [B1 ; dB] (FP32 or
-// split-BF16 arithmetic); then C += C~ + E in FP64.  FP64 appears only in the
+// one product with depth 2k, [dA | A1] * [B1 ; dB] (FP32 inputs with FP32 or
+// split-BF16 arithmetic, or -- when the main product is BF16 -- BF16 inputs:
+// E is 2^-8 of C, so BF16 rounding of dA, dB costs ~2^-17 of C); then
+// C += C~ + E in FP64.  FP64 appears only in the
 // split (A - A1 is formed in FP64) and in this final update.
 #pragma once
 #include "fmm_host.hpp"
@@ -35,38 +37,39 @@ __device__ __forceinline__ void round_in(double x, typename InT<IN>::T* out, flo
 }
 
 // A (m x k, FP64) -> A1 (m x k, type IN, ld la1) and AA = [dA | A1] (m x 2k,
-// FP32, ld laa).  One thread per element, column-major walk.
-template <int IN>
+// type CI = the correction's input type, FP32 or BF16; ld laa).  A1 is
+// exact in CI (CI is BF16 only when IN is).  One thread per element.
+template <int IN, int CI>
 __global__ void __launch_bounds__(kRefThreads) split_a_kernel(int m, int k, const double* A, long long lda,
-                                                              typename InT<IN>::T* A1, long long la1, float* AA,
-                                                              long long laa) {
+                                                              typename InT<IN>::T* A1, long long la1,
+                                                              typename InT<CI>::T* AA, long long laa) {
   const long long total = (long long)m * k;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(t % m), p = (int)(t / m);
     const double a = A[i + p * lda];
-    float a1f;
+    float a1f, unused;
     round_in<IN>(a, A1 + i + p * la1, &a1f);
-    AA[i + p * laa] = (float)(a - (double)a1f);
-    AA[i + (p + (long long)k) * laa] = a1f;
+    round_in<CI>(a - (double)a1f, AA + i + p * laa, &unused);
+    round_in<CI>((double)a1f, AA + i + (p + (long long)k) * laa, &unused);
   }
 }
 
 // B (k x n, FP64) -> B1 (k x n, type IN, ld lb1) and BB = [B1 ; dB] (2k x n,
-// FP32, ld lbb).
-template <int IN>
+// type CI, ld lbb).
+template <int IN, int CI>
 __global__ void __launch_bounds__(kRefThreads) split_b_kernel(int k, int n, const double* B, long long ldb,
-                                                              typename InT<IN>::T* B1, long long lb1, float* BB,
-                                                              long long lbb) {
+                                                              typename InT<IN>::T* B1, long long lb1,
+                                                              typename InT<CI>::T* BB, long long lbb) {
   const long long total = (long long)k * n;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
     const int p = (int)(t % k), j = (int)(t / k);
     const double b = B[p + j * ldb];
-    float b1f;
+    float b1f, unused;
     round_in<IN>(b, B1 + p + j * lb1, &b1f);
-    BB[p + j * lbb] = b1f;
-    BB[p + (long long)k + j * lbb] = (float)(b - (double)b1f);
+    round_in<CI>((double)b1f, BB + p + j * lbb, &unused);
+    round_in<CI>(b - (double)b1f, BB + p + (long long)k + j * lbb, &unused);
   }
 }
 
@@ -98,9 +101,10 @@ struct RefLayout {
 
 inline int in_size(int dtype) { return (dtype == kBF16 || dtype == kF16) ? 2 : 4; }
 
-inline RefLayout ref_layout(int main_dtype, int m, int n, int k, size_t ws_main, size_t ws_corr) {
+inline RefLayout ref_layout(int main_dtype, int corr_dtype, int m, int n, int k, size_t ws_main, size_t ws_corr) {
   RefLayout L{};
   const int s = in_size(main_dtype);
+  const int sc = in_size(corr_dtype);
   L.la1 = round8(std::max(m, 1));
   L.lb1 = round8(std::max(k, 1));
   L.laa = round8(std::max(m, 1));
@@ -114,8 +118,8 @@ inline RefLayout ref_layout(int main_dtype, int m, int n, int k, size_t ws_main,
   };
   L.a1 = take((size_t)L.la1 * k * s);
   L.b1 = take((size_t)L.lb1 * n * s);
-  L.aa = take((size_t)L.laa * 2 * k * 4);
-  L.bb = take((size_t)L.lbb * n * 4);
+  L.aa = take((size_t)L.laa * 2 * k * sc);
+  L.bb = take((size_t)L.lbb * n * sc);
   L.ct = take((size_t)L.lt * n * 4);
   L.et = take((size_t)L.lt * n * 4);
   L.wsm = take(ws_main);

@@ -168,7 +168,9 @@ int fmm_batch_destroy(fmm_batch_t batch);
  * remainders dA = fl32(A - A1), dB = fl32(B - B1),
  *     C~ = A1*B1              by main_plan (NULL: plain GEMM), FP32 result;
  *     E  = [dA | A1]*[B1; dB] by corr_plan (NULL: plain GEMM) of corr_dtype
- *                             (FMM_F32 or FMM_BF16X3), depth 2k, FP32 result;
+ *                             (FMM_F32 or FMM_BF16X3; FMM_BF16 when main_dtype
+ *                             is FMM_BF16, with dA, dB rounded to BF16),
+ *                             depth 2k, FP32 result;
  *     C += C~ + E             in FP64.
  * E is the residual A*B - C~ up to dA*dB and C~'s own arithmetic error, so
  * the correction removes the input-rounding error of the main product.  ws

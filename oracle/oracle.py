@@ -236,17 +236,19 @@ def fmm_lowp(alg, A: np.ndarray, B: np.ndarray, C: np.ndarray, dtype: str) -> np
 
 def refine_lowp(alg_main, main: str, alg_corr, corr: str, A, B, C) -> np.ndarray:
     """numpy restatement of the FP64 refinement: A1 = fl_main(A),
-    dA = fl32(A - A1) (B likewise), C~ = A1 B1 by the low-precision FMM of
+    dA = fl32(A - A1), rounded on to BF16 for a BF16 correction (B likewise),
+    C~ = A1 B1 by the low-precision FMM of
     alg_main (None: plain product), E = [dA | A1] [B1; dB] by alg_corr in
     `corr` arithmetic, result C + C~ + E in float64."""
     A = np.asarray(A, np.float64)
     B = np.asarray(B, np.float64)
     C = np.asarray(C, np.float64)
     inp = "f32" if main in ("f32", "bf16x3") else main
+    cin = "f32" if corr in ("f32", "bf16x3") else corr
     A1 = round_op(A.astype(np.float32), inp)
     B1 = round_op(B.astype(np.float32), inp)
-    dA = (A - A1.astype(np.float64)).astype(np.float32)
-    dB = (B - B1.astype(np.float64)).astype(np.float32)
+    dA = round_op((A - A1.astype(np.float64)).astype(np.float32), cin)
+    dB = round_op((B - B1.astype(np.float64)).astype(np.float32), cin)
     m, n = C.shape
     z = np.zeros((m, n), np.float32)
     Ct = (fmm_lowp(alg_main, A1, B1, z, main) if alg_main is not None
