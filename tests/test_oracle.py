@@ -137,3 +137,98 @@ def test_float_error_within_stated_tolerance(fmm, oracle, spec):
     got = C0.copy(order="F")
     oracle.fmm(alg, A, B, got, "abc")
     assert norm_err(got, A, B, C0) <= tol_for(len(alg.levels))
+
+
+# --- the numpy restatements of the low-precision kernels and the refinement ---
+
+def test_round_bf16_known_answers(oracle):
+    f = np.float32
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -7 + 2 ** -9, -(1.0 + 2 ** -8),
+                  3.0e38, 2 ** -130, 0.1], dtype=f)
+    want = np.array([1.0, 1.0, 1.0 + 2 ** -6, 1.0 + 2 ** -7, -1.0,
+                     3.0e38, 2 ** -130, 0.10009765625], dtype=f)
+    got = oracle.round_bf16(x)
+    # ties go to even (1 + 2^-8 -> 1, 1 + 3*2^-8 -> 1 + 2^-6); 2^-7 + 2^-9 rounds down;
+    # 3e38 -> the nearest BF16 (0x7f62 = 3.0040e38); 2^-130 is exact; 0.1 -> 0x3dcd
+    want[5] = np.frombuffer(np.uint32(0x7F620000).tobytes(), dtype=f)[0]
+    assert np.array_equal(got, want), (got, want)
+    # every value it returns is representable: the low 16 bits are zero
+    r = oracle.round_bf16(np.random.default_rng(0).standard_normal(10000).astype(f))
+    assert not np.any(r.view(np.uint32) & 0xFFFF)
+    # and it is the nearest one: no BF16 neighbour is closer
+    y = np.random.default_rng(1).uniform(-4, 4, 10000).astype(f)
+    ry = oracle.round_bf16(y).astype(np.float64)
+    ulp = 2.0 ** (np.floor(np.log2(np.abs(ry))) - 7)
+    assert np.all(np.abs(y - ry) <= ulp / 2 + 1e-30)
+
+
+def test_round_op_f16_matches_numpy_half(oracle):
+    x = np.random.default_rng(2).uniform(-10, 10, 1000).astype(np.float32)
+    assert np.array_equal(oracle.round_op(x, "f16"), x.astype(np.float16).astype(np.float32))
+    assert np.array_equal(oracle.round_op(x, "f32"), x)
+
+
+def test_split_bf16_product_is_compensated(oracle):
+    # S_hi T_hi + S_hi T_lo + S_lo T_hi recovers the FP32 product to ~2^-16
+    rng = np.random.default_rng(3)
+    S = rng.uniform(-1, 1, (64, 256)).astype(np.float32)
+    T = rng.uniform(-1, 1, (256, 48)).astype(np.float32)
+    exact = S.astype(np.float64) @ T.astype(np.float64)
+    e3 = np.abs(oracle._lowp_product(S, T, "bf16x3") - exact).max() / np.abs(exact).max()
+    e1 = np.abs(oracle._lowp_product(S, T, "bf16") - exact).max() / np.abs(exact).max()
+    assert e3 < 2 ** -14 < e1
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "f16", "bf16x3"])
+@pytest.mark.parametrize("spec", [(["222"], [1]), (["222"], [2]), (["333"], [1]), (["222", "323"], [1, 1])],
+                         ids=["222L1", "222L2", "333L1", "222x323"])
+def test_fmm_lowp_integer_exact(fmm, oracle, spec, dtype):
+    # small integers: every operand sum and partial sum is exact in every type
+    alg = fmm.build_algorithm(*spec)
+    Mx, Kx, Nx = alg.dims
+    rng = np.random.default_rng(4)
+    m, n, k = Mx * 8 * 3 + 5, Nx * 4 * 3 + 3, Kx * 7 + 1
+    A = rng.integers(-4, 5, (m, k)).astype(np.float32)
+    B = rng.integers(-4, 5, (k, n)).astype(np.float32)
+    C = rng.integers(-4, 5, (m, n)).astype(np.float32)
+    got = oracle.fmm_lowp(alg, A, B, C, dtype)
+    assert np.array_equal(got, C.astype(np.float64) + A.astype(np.float64) @ B.astype(np.float64))
+
+
+def test_fmm_lowp_planted_mistakes_are_caught(fmm, oracle):
+    # a flipped W sign in the table must change the integer result
+    alg = fmm.build_algorithm(["222"], [1])
+    rng = np.random.default_rng(5)
+    A = rng.integers(-4, 5, (32, 18)).astype(np.float32)
+    B = rng.integers(-4, 5, (18, 16)).astype(np.float32)
+    C = np.zeros((32, 16), np.float32)
+    good = oracle.fmm_lowp(alg, A, B, C, "bf16")
+    alg.W[0][0] = -alg.W[0][0]
+    try:
+        bad = oracle.fmm_lowp(alg, A, B, C, "bf16")
+    finally:
+        alg.W[0][0] = -alg.W[0][0]
+    assert not np.array_equal(bad, good)
+
+
+def test_refine_restatement_integer_and_float(fmm, oracle):
+    rng = np.random.default_rng(6)
+    m, n, k = 40, 24, 18
+    A = rng.integers(-4, 5, (m, k)).astype(np.float64)
+    B = rng.integers(-4, 5, (k, n)).astype(np.float64)
+    C = rng.integers(-4, 5, (m, n)).astype(np.float64)
+    for main, corr in (("bf16", "f32"), ("bf16", "bf16"), ("f16", "f32"), ("bf16x3", "bf16x3")):
+        assert np.array_equal(oracle.refine_lowp(None, main, None, corr, A, B, C), C + A @ B)
+    A = rng.uniform(-1, 1, (m, k))
+    B = rng.uniform(-1, 1, (k, n))
+    exact = C + A @ B
+    plain = C + oracle._lowp_product(oracle.round_op(A.astype(np.float32), "bf16"),
+                                     oracle.round_op(B.astype(np.float32), "bf16"), "bf16")
+    ref = oracle.refine_lowp(None, "bf16", None, "f32", A, B, C)
+    e_plain = np.linalg.norm(plain - exact) / np.linalg.norm(exact)
+    e_ref = np.linalg.norm(ref - exact) / np.linalg.norm(exact)
+    assert e_ref < e_plain / 100       # the correction restores the input rounding
+    # a dropped correction term (dB) is caught
+    A1 = oracle.round_op(A.astype(np.float32), "bf16").astype(np.float64)
+    half = C + A1 @ oracle.round_op(B.astype(np.float32), "bf16").astype(np.float64) + (A - A1) @ B
+    assert np.linalg.norm(half - ref) / np.linalg.norm(exact) > 10 * e_ref
