@@ -308,6 +308,22 @@ inline int batch_specs(int nprob, const fmm_problem_t* probs, int nplans, const 
 
 extern "C" {
 
+int fmm_timing(int enable) {
+  fmm::LaunchTiming& t = fmm::launch_timing();
+  if (enable && !t.ev[0]) {
+    if (hipEventCreate(&t.ev[0]) != hipSuccess || hipEventCreate(&t.ev[1]) != hipSuccess) return fmm::kEHip;
+  }
+  t.on = enable != 0;
+  return fmm::kOK;
+}
+
+int fmm_timing_last_ms(float* ms) {
+  fmm::LaunchTiming& t = fmm::launch_timing();
+  if (!ms || !t.ev[0]) return fmm::kEInval;
+  if (hipEventSynchronize(t.ev[1]) != hipSuccess) return fmm::kEHip;
+  return hipEventElapsedTime(ms, t.ev[0], t.ev[1]) == hipSuccess ? fmm::kOK : fmm::kEHip;
+}
+
 int fmm_batch_create(fmm_batch_t* out, int dtype, int nprob, const fmm_problem_t* probs, int nplans,
                      const fmm_plan_t* plans) {
   if (!out) return fmm::kEInval;

@@ -539,6 +539,18 @@ inline size_t ws_ints(const JobList& L) { return L.ordered ? (size_t)(kWsCounter
 // Launch a job list (<= kInlineJobs jobs, passed by value) on `stream` with
 // the persistent kernel `kern` (nt threads per workgroup).  `ws` must hold
 // ws_ints(L) ints when L.ordered.
+// Optional timing of product launches (for a benchmark's roofline of the
+// kernel itself): when enabled, the launch of each product kernel on the
+// calling thread is bracketed by two HIP events; the last pair is kept.
+struct LaunchTiming {
+  bool on = false;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+inline LaunchTiming& launch_timing() {
+  static thread_local LaunchTiming t;
+  return t;
+}
+
 inline int launch_jobs_with(void (*kern)(Launch), int nt, JobList JL, int* ws, hipStream_t stream) {
   if (JL.total_units == 0) return kOK;
   if ((int)JL.jobs.size() > kInlineJobs) return kEInval;
@@ -572,8 +584,11 @@ inline int launch_jobs_with(void (*kern)(Launch), int nt, JobList JL, int* ws, h
     L.nq = kMaxQueues;
     L.local_handover = FMM_LOCAL_HANDOVER && JL.ordered && di.xcd_round_robin ? 1 : 0;
   }
+  LaunchTiming& tm = launch_timing();
+  if (tm.on && tm.ev[0]) (void)hipEventRecord(tm.ev[0], stream);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(nt), 0, stream, L);
   if (hipGetLastError() != hipSuccess) return kEHip;
+  if (tm.on && tm.ev[1]) (void)hipEventRecord(tm.ev[1], stream);
   return kOK;
 }
 

@@ -33,7 +33,8 @@ EXPORTS = ("fmm_plan_create", "fmm_plan_destroy", "fmm_plan_info", "fmm_workspac
            "fmm_ws_status", "fmm_host_abc", "fmm_host_run", "bl_dgemm", "bl_dgemm_ref",
            "bl_finalize", "fmm_batch_create", "fmm_batch_set", "fmm_batch_run", "fmm_batch_variant",
            "fmm_batch_info", "fmm_batch_status", "fmm_batch_destroy", "fmm_batch_reserve",
-           "fmm_batch_generation", "fmm_refine", "fmm_refine_workspace_bytes")
+           "fmm_batch_generation", "fmm_refine", "fmm_refine_workspace_bytes", "fmm_timing",
+           "fmm_timing_last_ms")
 
 
 class FmmError(RuntimeError):
@@ -411,3 +412,21 @@ def refine(A, B, C, main: str = "bf16", main_plan: Optional[Plan] = None, corr: 
     check(L.fmm_refine(mp, DTYPE_CODES[main], cp, DTYPE_CODES[corr], m, n, k, A.data_ptr(), _ld(A),
                        B.data_ptr(), _ld(B), C.data_ptr(), _ld(C), None, 0, _stream_ptr(stream)),
           f"fmm_refine({main}, {corr})")
+
+
+def kernel_timing(enable: bool, library=None) -> None:
+    """Bracket every product-kernel launch of this thread with HIP events."""
+    L = library or lib()
+    L.fmm_timing.argtypes = [ct.c_int]
+    L.fmm_timing.restype = ct.c_int
+    check(L.fmm_timing(1 if enable else 0), "fmm_timing")
+
+
+def last_kernel_ms(library=None) -> float:
+    """Duration of the last timed product-kernel launch (waits for it)."""
+    L = library or lib()
+    L.fmm_timing_last_ms.argtypes = [ct.POINTER(ct.c_float)]
+    L.fmm_timing_last_ms.restype = ct.c_int
+    v = ct.c_float(0.0)
+    check(L.fmm_timing_last_ms(ct.byref(v)), "fmm_timing_last_ms")
+    return float(v.value)

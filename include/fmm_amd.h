@@ -117,6 +117,14 @@ int fmm_gemm(int dtype, int m, int n, int k, const void *A, int lda,
 /* Error flags a launch left in ws (synchronises the device first). */
 int fmm_ws_status(const void *ws, int *flags);
 
+/* Kernel timing for benchmarks (no counterpart in the reference): while
+ * enabled, the product-kernel launch of every call made on this host thread
+ * is bracketed by two HIP events on its stream; fmm_timing_last_ms waits for
+ * the last pair and returns its elapsed time (the kernel alone, without the
+ * workspace fill, pre-sum or M_Add launches of the same call). */
+int fmm_timing(int enable);
+int fmm_timing_last_ms(float *ms);
+
 /* ---- batches: many independent problems in one persistent launch ----------
  * No counterpart in the reference (one FMM call per entry there).  A batch is
  * built from a problem list (device pointers; each problem C += A*B with its
