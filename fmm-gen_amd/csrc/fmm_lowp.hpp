@@ -694,8 +694,8 @@ template <class C>
 __device__ __forceinline__ void l_unit(const Launch& L, const Job& J, int q, int p, char* lds, int tid) {
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % C::WM, wn = wave / C::WM;
-  const int tmi = p / J.tiles_n;
-  const int tni = p - tmi * J.tiles_n;
+  int tmi, tni;
+  tile_coords<4>(J.tiles_m, J.tiles_n, p, &tmi, &tni);
   LGeo<C> g;
   g.row0 = tmi * C::BM;
   g.col0 = tni * C::BN;
