@@ -32,14 +32,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#ifndef FMM_PINGPONG
-#define FMM_PINGPONG 1
-#endif
 #ifndef FMM_SETPRIO
 #define FMM_SETPRIO 1
-#endif
-#ifndef FMM_FRAG_PIPE
-#define FMM_FRAG_PIPE 1
 #endif
 
 namespace fmm {
@@ -399,59 +393,72 @@ __device__ __forceinline__ void store_slice(double (*As)[BM], double (*Bs)[TileK
 // j+16k and element v of lane l holding D[l/16 + 4v][l%16] (layout measured
 // on gfx950).  X = B^T and Y = A^T, so D is the transposed C block: element v
 // of lane l is C row l%16, column l/16 + 4v.
-template <int BK, int KK0, int KK1>
+//
+// Scheduling per kernel variant (FP64 measurements, 9600^3): the single-term
+// kernel (CH = 1; plain GEMMs, Naive products) reads the fragments of step
+// kk+1 while the MFMAs of step kk issue and runs the two waves of a SIMD in
+// opposite phase order (pingpong, see contract); the operand-sum kernels have
+// too few free registers for either and run faster without both (ABC 333:
+// 49.6 -> 51.4 TFLOP/s).
+template <int CH>
+struct F64Sched {
+  static constexpr bool frag_pipe = CH == 1;
+  static constexpr bool pingpong = CH == 1;
+};
+
+template <int BK, int KK0, int KK1, bool FRAG_PIPE>
 __device__ __forceinline__ void mfma_slice(const double (*As)[BM], const double (*Bs)[TileK<BK>::BKP], int lane, int wm,
                                            int wn, f64x4 (&acc)[FM][FN]) {
   const int l16 = lane & 15, lk = lane >> 4;
 #if FMM_SETPRIO
   __builtin_amdgcn_s_setprio(1);
 #endif
-#if FMM_FRAG_PIPE
-  // Fragments of step kk+1 are read while the MFMAs of step kk issue.
-  double a[2][FM], b[2][FN];
+  if constexpr (FRAG_PIPE) {
+    // Fragments of step kk+1 are read while the MFMAs of step kk issue.
+    double a[2][FM], b[2][FN];
 #pragma unroll
-  for (int mi = 0; mi < FM; ++mi) a[0][mi] = As[KK0 * 4 + lk][wm * WTM + mi * 16 + l16];
+    for (int mi = 0; mi < FM; ++mi) a[0][mi] = As[KK0 * 4 + lk][wm * WTM + mi * 16 + l16];
 #pragma unroll
-  for (int ni = 0; ni < FN; ++ni) b[0][ni] = Bs[wn * WTN + ni * 16 + l16][KK0 * 4 + lk];
+    for (int ni = 0; ni < FN; ++ni) b[0][ni] = Bs[wn * WTN + ni * 16 + l16][KK0 * 4 + lk];
 #pragma unroll
-  for (int kk = KK0; kk < KK1; ++kk) {
-    const int c = (kk - KK0) & 1;
-    if (kk + 1 < KK1) {
+    for (int kk = KK0; kk < KK1; ++kk) {
+      const int c = (kk - KK0) & 1;
+      if (kk + 1 < KK1) {
 #pragma unroll
-      for (int mi = 0; mi < FM; ++mi) a[c ^ 1][mi] = As[(kk + 1) * 4 + lk][wm * WTM + mi * 16 + l16];
+        for (int mi = 0; mi < FM; ++mi) a[c ^ 1][mi] = As[(kk + 1) * 4 + lk][wm * WTM + mi * 16 + l16];
 #pragma unroll
-      for (int ni = 0; ni < FN; ++ni) b[c ^ 1][ni] = Bs[wn * WTN + ni * 16 + l16][(kk + 1) * 4 + lk];
-    }
-#pragma unroll
-    for (int mi = 0; mi < FM; ++mi)
-#pragma unroll
-      for (int ni = 0; ni < FN; ++ni)
-        acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[c][ni], a[c][mi], acc[mi][ni], 0, 0, 0);
-    if (kk + 1 < KK1) {
-      // interleave: one LDS read after every two MFMAs
-#pragma unroll
-      for (int x = 0; x < (FM + FN) / 2; ++x) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
-        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        for (int ni = 0; ni < FN; ++ni) b[c ^ 1][ni] = Bs[wn * WTN + ni * 16 + l16][(kk + 1) * 4 + lk];
       }
-      __builtin_amdgcn_sched_group_barrier(0x008, FM * FN, 0);
+#pragma unroll
+      for (int mi = 0; mi < FM; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < FN; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[c][ni], a[c][mi], acc[mi][ni], 0, 0, 0);
+      if (kk + 1 < KK1) {
+        // interleave: one LDS read after every two MFMAs
+#pragma unroll
+        for (int x = 0; x < (FM + FN) / 2; ++x) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, FM * FN, 0);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int kk = KK0; kk < KK1; ++kk) {
+      double a[FM], b[FN];
+#pragma unroll
+      for (int mi = 0; mi < FM; ++mi) a[mi] = As[kk * 4 + lk][wm * WTM + mi * 16 + l16];
+#pragma unroll
+      for (int ni = 0; ni < FN; ++ni) b[ni] = Bs[wn * WTN + ni * 16 + l16][kk * 4 + lk];
+#pragma unroll
+      for (int mi = 0; mi < FM; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < FN; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi][ni], 0, 0, 0);
     }
   }
-#else
-#pragma unroll
-  for (int kk = KK0; kk < KK1; ++kk) {
-    double a[FM], b[FN];
-#pragma unroll
-    for (int mi = 0; mi < FM; ++mi) a[mi] = As[kk * 4 + lk][wm * WTM + mi * 16 + l16];
-#pragma unroll
-    for (int ni = 0; ni < FN; ++ni) b[ni] = Bs[wn * WTN + ni * 16 + l16][kk * 4 + lk];
-#pragma unroll
-    for (int mi = 0; mi < FM; ++mi)
-#pragma unroll
-      for (int ni = 0; ni < FN; ++ni)
-        acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi][ni], 0, 0, 0);
-  }
-#endif
 #if FMM_SETPRIO
   __builtin_amdgcn_s_setprio(0);
 #endif
@@ -549,15 +556,13 @@ __device__ __forceinline__ void scatter_c(const Job& J, const Prod& P, int row0,
 }
 
 // Contraction of one unit over its whole depth into acc, double-buffered in
-// LDS: while the MFMAs of slice st run, slice st+1 (loaded one slice earlier)
-// is summed and stored into the other buffer and the loads of slice st+2 are
-// issued; one barrier per slice.
-// One slice of the pipelined contraction: MFMAs on LDS buffer `cur` while
-// slice st+1 (loaded one slice earlier) is summed into the other buffer and
-// the loads of slice st+2 are issued.  The two waves that share a SIMD run
-// the phases in opposite order (PINGPONG): wave group 0 does MFMA half 1,
-// sums, MFMA half 2; group 1 does the sums first, so one wave's VALU work and
-// load waits overlap the other wave's MFMAs instead of idling the matrix pipe.
+// LDS: while the MFMAs of slice st run on buffer `cur`, slice st+1 (loaded one
+// slice earlier) is summed and stored into the other buffer and the loads of
+// slice st+2 are issued; one barrier per slice.  A wave does MFMA half 1,
+// sums, MFMA half 2.  With pingpong (F64Sched) the second wave of each SIMD
+// (waves w and w+4 share one, measured) does the sums first and then all
+// MFMAs, so one wave's VALU work and load waits overlap the other wave's
+// MFMAs.
 template <int BK, int CH>
 __device__ __forceinline__ void sums_phase(const Job& J, const Prod& P, const Srcs<CH>& S, int st, int nsteps,
                                            const UnitGeo& g, Raw<BK, CH>& r, Slice<BK>& s, double (*As)[BK][BM],
@@ -584,23 +589,17 @@ __device__ __forceinline__ void contract(const Job& J, const Prod& P, const Unit
   store_slice<BK>(As[0], Bs[0], g, s);
   if (nsteps > 1) load_raw<BK, CH>(J, S, BK, g, r);
   __syncthreads();
+  constexpr bool FP = F64Sched<CH>::frag_pipe;
   for (int st = 0; st < nsteps; ++st) {
     const int cur = st & 1;
-#if FMM_PINGPONG
-    if (grp == 0) {
-      mfma_slice<BK, 0, BK / 8>(As[cur], Bs[cur], lane, wm, wn, acc);
+    if (F64Sched<CH>::pingpong && grp != 0) {
       sums_phase<BK, CH>(J, P, S, st, nsteps, g, r, s, As, Bs, cur ^ 1);
-      mfma_slice<BK, BK / 8, BK / 4>(As[cur], Bs[cur], lane, wm, wn, acc);
+      mfma_slice<BK, 0, BK / 4, FP>(As[cur], Bs[cur], lane, wm, wn, acc);
     } else {
+      mfma_slice<BK, 0, BK / 8, FP>(As[cur], Bs[cur], lane, wm, wn, acc);
       sums_phase<BK, CH>(J, P, S, st, nsteps, g, r, s, As, Bs, cur ^ 1);
-      mfma_slice<BK, 0, BK / 4>(As[cur], Bs[cur], lane, wm, wn, acc);
+      mfma_slice<BK, BK / 8, BK / 4, FP>(As[cur], Bs[cur], lane, wm, wn, acc);
     }
-#else
-    (void)grp;
-    mfma_slice<BK, 0, BK / 8>(As[cur], Bs[cur], lane, wm, wn, acc);
-    sums_phase<BK, CH>(J, P, S, st, nsteps, g, r, s, As, Bs, cur ^ 1);
-    mfma_slice<BK, BK / 8, BK / 4>(As[cur], Bs[cur], lane, wm, wn, acc);
-#endif
     __syncthreads();
   }
 }
