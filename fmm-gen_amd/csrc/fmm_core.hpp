@@ -32,9 +32,6 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#ifndef FMM_SETPRIO
-#define FMM_SETPRIO 1
-#endif
 
 namespace fmm {
 
@@ -397,22 +394,23 @@ __device__ __forceinline__ void store_slice(double (*As)[BM], double (*Bs)[TileK
 // Scheduling per kernel variant (FP64 measurements, 9600^3): the single-term
 // kernel (CH = 1; plain GEMMs, Naive products) reads the fragments of step
 // kk+1 while the MFMAs of step kk issue and runs the two waves of a SIMD in
-// opposite phase order (pingpong, see contract); the operand-sum kernels have
-// too few free registers for either and run faster without both (ABC 333:
-// 49.6 -> 51.4 TFLOP/s).
+// opposite phase order (pingpong, see contract), at equal wave priority
+// (raising it for the MFMA phases cost 1%); the operand-sum kernels have too
+// few free registers for either and run faster without both (ABC 333: 49.6
+// -> 51.4 TFLOP/s), but gain from the raised priority (s_setprio: 49.6 ->
+// 51.5).
 template <int CH>
 struct F64Sched {
   static constexpr bool frag_pipe = CH == 1;
   static constexpr bool pingpong = CH == 1;
+  static constexpr bool setprio = CH != 1;
 };
 
-template <int BK, int KK0, int KK1, bool FRAG_PIPE>
+template <int BK, int KK0, int KK1, bool FRAG_PIPE, bool SETPRIO>
 __device__ __forceinline__ void mfma_slice(const double (*As)[BM], const double (*Bs)[TileK<BK>::BKP], int lane, int wm,
                                            int wn, f64x4 (&acc)[FM][FN]) {
   const int l16 = lane & 15, lk = lane >> 4;
-#if FMM_SETPRIO
-  __builtin_amdgcn_s_setprio(1);
-#endif
+  if constexpr (SETPRIO) __builtin_amdgcn_s_setprio(1);
   if constexpr (FRAG_PIPE) {
     // Fragments of step kk+1 are read while the MFMAs of step kk issue.
     double a[2][FM], b[2][FN];
@@ -459,9 +457,7 @@ __device__ __forceinline__ void mfma_slice(const double (*As)[BM], const double 
           acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi][ni], 0, 0, 0);
     }
   }
-#if FMM_SETPRIO
-  __builtin_amdgcn_s_setprio(0);
-#endif
+  if constexpr (SETPRIO) __builtin_amdgcn_s_setprio(0);
 }
 
 // Hand-over of a C tile between consecutive product units.
@@ -589,16 +585,16 @@ __device__ __forceinline__ void contract(const Job& J, const Prod& P, const Unit
   store_slice<BK>(As[0], Bs[0], g, s);
   if (nsteps > 1) load_raw<BK, CH>(J, S, BK, g, r);
   __syncthreads();
-  constexpr bool FP = F64Sched<CH>::frag_pipe;
+  constexpr bool FP = F64Sched<CH>::frag_pipe, PR = F64Sched<CH>::setprio;
   for (int st = 0; st < nsteps; ++st) {
     const int cur = st & 1;
     if (F64Sched<CH>::pingpong && grp != 0) {
       sums_phase<BK, CH>(J, P, S, st, nsteps, g, r, s, As, Bs, cur ^ 1);
-      mfma_slice<BK, 0, BK / 4, FP>(As[cur], Bs[cur], lane, wm, wn, acc);
+      mfma_slice<BK, 0, BK / 4, FP, PR>(As[cur], Bs[cur], lane, wm, wn, acc);
     } else {
-      mfma_slice<BK, 0, BK / 8, FP>(As[cur], Bs[cur], lane, wm, wn, acc);
+      mfma_slice<BK, 0, BK / 8, FP, PR>(As[cur], Bs[cur], lane, wm, wn, acc);
       sums_phase<BK, CH>(J, P, S, st, nsteps, g, r, s, As, Bs, cur ^ 1);
-      mfma_slice<BK, BK / 8, BK / 4, FP>(As[cur], Bs[cur], lane, wm, wn, acc);
+      mfma_slice<BK, BK / 8, BK / 4, FP, PR>(As[cur], Bs[cur], lane, wm, wn, acc);
     }
     __syncthreads();
   }
