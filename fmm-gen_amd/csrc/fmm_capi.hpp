@@ -66,7 +66,7 @@ inline int run_fmm(Plan* P, int pack, int dtype, int m, int n, int k, const void
   if (!dtype_geo(dtype, &G)) return kEUnsupported;
   if (m == 0 || n == 0 || k == 0) return kOK;
   PackLayout Lo;
-  if (pack == kPackABC || !pack_layout(*P, pack, m, n, k, &Lo, G.sin, G.sc))
+  if (pack == kPackABC || !pack_layout(*P, pack, m, n, k, &Lo, G.sin, G.sc, naive_fused(dtype)))
     return run_abc(P, dtype, m, n, k, A, lda, B, ldb, C, ldc, ws, ws_bytes, s);
   void* w = ws;
   bool own = false;

@@ -87,8 +87,11 @@ struct Plan {
   bool unit_single = true; // every product: one A and one B term, coefficient +-1
   // ABC: R products, then Mx*Nx k-fringe products.
   Table main;
-  // Naive: product r reads workspace blocks S_r, T_r and writes M_r.
+  // Naive: product r reads workspace blocks S_r, T_r and writes M_r (FP64)...
   Table diag;
+  // ... or updates its C blocks (the W terms of `main`) straight from the
+  // accumulators, in product order per C tile, as ABC (other types).
+  Table naive;
   // AB: product r reads its A/B terms as in ABC and writes M_r.
   Table ab;
   // For every C block (I*Nx + J, grid order) the products feeding it, in
@@ -184,10 +187,14 @@ inline int build_plan(Plan* P, int nlev, const int* dims, int R, const double* U
     }
   // Naive and AB tables.  The workspaces hold one leaf block per product,
   // side by side: S_r, T_r and M_r are block (0, r) of S (ld >= ms), T
-  // (ld >= ks) and M (ld >= ms).  The Mx*Nx k-fringe products follow as
-  // products R + I*Nx + J, reading A and B and writing M block R + I*Nx + J.
+  // (ld >= ks) and M (ld >= ms).  In the M-writing tables the Mx*Nx k-fringe
+  // products follow as products R + I*Nx + J, reading A and B and writing M
+  // block R + I*Nx + J; the fused Naive launch runs the k-fringe products of
+  // `main` in a launch of their own.
   P->diag.prods.h.clear();
   P->diag.terms.h.clear();
+  P->naive.prods.h.clear();
+  P->naive.terms.h.clear();
   P->ab.prods.h.clear();
   P->ab.terms.h.clear();
   for (int r = 0; r < R; ++r) {
@@ -203,6 +210,19 @@ inline int build_plan(Plan* P, int nlev, const int* dims, int R, const double* U
     dt.push_back(Term{1.0, 0, r});
     d.nc = 1;
     P->diag.prods.h.push_back(d);
+
+    std::vector<Term>& ft = P->naive.terms.h;
+    Prod f{};
+    f.a0 = (int)ft.size();
+    ft.push_back(Term{1.0, 0, r});
+    f.na = 1;
+    f.b0 = (int)ft.size();
+    ft.push_back(Term{1.0, 0, r});
+    f.nb = 1;
+    f.c0 = (int)ft.size();
+    for (int t = 0; t < prods[r].nc; ++t) ft.push_back(terms[prods[r].c0 + t]);
+    f.nc = prods[r].nc;
+    P->naive.prods.h.push_back(f);
 
     std::vector<Term>& at = P->ab.terms.h;
     const Prod& o = prods[r];
@@ -279,7 +299,7 @@ inline int device_clists(Plan* P, int** ptr, int** prod, double** w) {
 }
 
 inline void release_plan(Plan* P) {
-  for (Table* T : {&P->main, &P->diag, &P->ab}) {
+  for (Table* T : {&P->main, &P->diag, &P->naive, &P->ab}) {
     T->prods.release();
     T->terms.release();
   }
@@ -695,9 +715,10 @@ enum Pack : int { kPackABC = 0, kPackAB = 1, kPackNaive = 2 };
 
 struct PackLayout {
   int md = 0, kd = 0, nd = 0, ms = 0, ks = 0, ns = 0, kr = 0;
-  int nlive = 0;                       // M blocks written (R [+ Mx*Nx])
+  int nlive = 0;                       // AB: M blocks written (R [+ Mx*Nx])
   long long ldm = 0, lds = 0, ldt = 0; // leading dimensions of M, S, T
-  size_t m_off = 0, s_off = 0, t_off = 0, bytes = 0;
+  size_t m_off = 0, s_off = 0, t_off = 0, o_off = 0, bytes = 0;
+  int o_ints = 0;                      // Naive: ints of the ordered launch's ws
 };
 
 inline size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -706,7 +727,14 @@ inline size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
 // core (then the call is the ABC path's plain GEMMs).
 // Element sizes: sin of A / B (and of the S, T sums), sc of C (and of M).
 // The default (FP64) layout is the largest, so its size bounds every dtype's.
-inline bool pack_layout(const Plan& P, int pack, int m, int n, int k, PackLayout* L, int sin = 8, int sc = 8) {
+// Naive with its C updates fused into the product launch (the low-precision
+// types, where it measured 5-20% faster than M + M_Add), or M + M_Add (FP64,
+// 2% faster: the FP64 product launch idles its CU for each C read-modify-write
+// of a unit, M_Add streams).
+inline bool naive_fused(int dtype) { return dtype != kF64; }
+
+inline bool pack_layout(const Plan& P, int pack, int m, int n, int k, PackLayout* L, int sin = 8, int sc = 8,
+                        bool fused = false) {
   PackLayout o;
   o.md = m - m % (P.Mx * kMR);
   o.kd = k - k % P.Kx;
@@ -721,13 +749,21 @@ inline bool pack_layout(const Plan& P, int pack, int m, int n, int k, PackLayout
   o.lds = o.ms;
   o.ldt = (o.ks + 7) & ~7;             // T columns stay 16-byte aligned for every type
   size_t off = 0;
-  o.m_off = off;
-  off += round256((size_t)sc * (size_t)o.ldm * o.ns * o.nlive);
   if (pack == kPackNaive) {
     o.s_off = off;
     off += round256((size_t)sin * (size_t)o.lds * o.ks * P.R);
     o.t_off = off;
     off += round256((size_t)sin * (size_t)o.ldt * o.ns * P.R);
+  }
+  if (pack == kPackNaive && fused) {
+    // order counters of the product launch, one per C tile of the smallest
+    // tile shape (128 x 128) of any type
+    o.o_off = off;
+    o.o_ints = kWsCounters + tiles(o.ms, 128) * tiles(o.ns, 128);
+    off += round256(sizeof(int) * (size_t)o.o_ints);
+  } else {
+    o.m_off = off;
+    off += round256((size_t)sc * (size_t)o.ldm * o.ns * o.nlive);
   }
   o.bytes = off;
   *L = o;
@@ -736,19 +772,169 @@ inline bool pack_layout(const Plan& P, int pack, int m, int n, int k, PackLayout
 
 inline int ew_grid(int rows, int cols) { return tiles(rows, kEwRows) * tiles(cols, kEwCols); }
 
-// One AB or Naive call C += A*B on `stream`: [presum A, presum B,] one product
-// launch (the products into M, the k-fringe products into M, the right and
-// bottom fringe GEMMs straight into C), then M_Add into the C core.  Lo must
-// be the layout of this dtype (pack_layout with its element sizes).
+// Naive pre-sums S_r = sum_t u_t A_t and T_r = sum_t v_t B_t into the
+// workspace blocks (0, r) of S and T, with the ABC kernel's packing arithmetic.
+inline int launch_presums(const Plan* P, int dtype, const PackLayout& Lo, const Prod* dp, const Term* dt,
+                          const void* A, long long lda, const void* B, long long ldb, void* S, void* T,
+                          hipStream_t stream) {
+  const dim3 ga(ew_grid(Lo.ms, Lo.ks), P->R), gb(ew_grid(Lo.ks, Lo.ns), P->R);
+  if (dtype == kF64) {
+    SumJob sa{};
+    sa.X = static_cast<const double*>(A);
+    sa.Y = static_cast<double*>(S);
+    sa.ldx = lda;
+    sa.ldy = Lo.lds;
+    sa.rows = Lo.ms;
+    sa.cols = Lo.ks;
+    sa.ny = 1;
+    sa.side = 0;
+    sa.vec = ((uintptr_t)A & 15) == 0 && (lda & 1) == 0 && ((uintptr_t)S & 15) == 0;
+    sa.prods = dp;
+    sa.terms = dt;
+    presum_kernel<<<ga, dim3(kEwThreads), 0, stream>>>(sa);
+    if (hipGetLastError() != hipSuccess) return kEHip;
+    SumJob sb = sa;
+    sb.X = static_cast<const double*>(B);
+    sb.Y = static_cast<double*>(T);
+    sb.ldx = ldb;
+    sb.ldy = Lo.ldt;
+    sb.rows = Lo.ks;
+    sb.cols = Lo.ns;
+    sb.side = 1;
+    sb.vec = ((uintptr_t)B & 15) == 0 && (ldb & 1) == 0 && (Lo.ks & 1) == 0 && ((uintptr_t)T & 15) == 0;
+    presum_kernel<<<gb, dim3(kEwThreads), 0, stream>>>(sb);
+  } else {
+    SumJobL sa{};
+    sa.X = A;
+    sa.Y = S;
+    sa.ldx = lda;
+    sa.ldy = Lo.lds;
+    sa.rows = Lo.ms;
+    sa.cols = Lo.ks;
+    sa.ny = 1;
+    sa.side = 0;
+    sa.prods = dp;
+    sa.terms = dt;
+    SumJobL sb = sa;
+    sb.X = B;
+    sb.Y = T;
+    sb.ldx = ldb;
+    sb.ldy = Lo.ldt;
+    sb.rows = Lo.ks;
+    sb.cols = Lo.ns;
+    sb.side = 1;
+    if (dtype == kBF16) {
+      presum_lowp_kernel<kInBF16><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
+      presum_lowp_kernel<kInBF16><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
+    } else if (dtype == kF16) {
+      presum_lowp_kernel<kInF16><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
+      presum_lowp_kernel<kInF16><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
+    } else {
+      presum_lowp_kernel<kInF32><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
+      presum_lowp_kernel<kInF32><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
+    }
+  }
+  if (hipGetLastError() != hipSuccess) return kEHip;
+  return kOK;
+}
+
+// One fused Naive call C += A*B on `stream`: the pre-sums, then one product launch
+// M_r = S_r T_r whose epilogue applies C_j = fma(w_jr, M_r, C_j) for every
+// C block of product r straight from the accumulators, in product order per
+// C tile (the ordered hand-over of the ABC kernel), with the right and bottom
+// fringe GEMMs as further jobs; then, when k has a fringe, the Mx*Nx k-fringe
+// products of `main` in a launch of their own.  The reference stores M_r and
+// adds it in a pass of its own (meta/naive_gen_mix.py:60-137); the C updates
+// and their order are the same, so the result equals ABC's bit for bit.
+inline int launch_naive_fused(Plan* P, int dtype, const PackLayout& Lo, int m, int n, int k, const void* A,
+                              long long lda, const void* B, long long ldb, void* C, long long ldc, void* ws,
+                              hipStream_t stream) {
+  KGeo G;
+  if (!dtype_geo(dtype, &G)) return kEUnsupported;
+  Prod *dp, *np, *ip;
+  Term *dt, *nt, *it;
+  int rc = device_tables(P, P->main, &dp, &dt);
+  if (rc == kOK) rc = device_tables(P, P->naive, &np, &nt);
+  if (rc == kOK) rc = device_tables(identity_plan(), &ip, &it);
+  if (rc != kOK) return rc;
+  char* w = static_cast<char*>(ws);
+  void* S = w + Lo.s_off;
+  void* T = w + Lo.t_off;
+  rc = launch_presums(P, dtype, Lo, dp, dt, A, lda, B, ldb, S, T, stream);
+  if (rc != kOK) return rc;
+  JobList JL;
+  add_gemm_job(JL, G, ip, it, m, n - Lo.nd, k, A, lda, eoff(B, (long long)Lo.nd * ldb, G.sin), ldb,
+               eoff(C, (long long)Lo.nd * ldc, G.sc), ldc);
+  add_gemm_job(JL, G, ip, it, m - Lo.md, Lo.nd, k, eoff(A, Lo.md, G.sin), lda, B, ldb, eoff(C, Lo.md, G.sc), ldc);
+  Job j{};
+  j.A = S;
+  j.B = T;
+  j.C = C;
+  j.lda = Lo.lds;
+  j.ldb = Lo.ldt;
+  j.ldc = ldc;
+  j.ms = Lo.ms;
+  j.ns = Lo.ns;
+  j.ks = Lo.ks;
+  j.kr = 0;
+  j.tiles_m = tiles(Lo.ms, G.bm);
+  j.tiles_n = tiles(Lo.ns, G.bn);
+  j.nprod = P->R;
+  j.gsize = 1;
+  j.ngroups = P->R;
+  j.unit_base = JL.total_units;
+  j.order_base = P->R > 1 ? JL.counters : -1;
+  j.flags = vec_flags(G, S, Lo.lds, T, Lo.ldt, 0, C, ldc);   // T_r is block (0, r): no row offset
+  j.prods = np;
+  j.terms = nt;
+  if (P->R > 1) {
+    JL.counters += j.tiles_m * j.tiles_n;
+    JL.ordered = true;
+  }
+  JL.total_units += j.nprod * j.tiles_m * j.tiles_n;
+  JL.jobs.push_back(j);
+  if (ws_ints(JL) > (size_t)Lo.o_ints) return kEInval;   // the layout reserves 128 x 128 tiles
+  rc = launch_jobs_dtype(dtype, JL, 1, reinterpret_cast<int*>(w + Lo.o_off), stream, true);
+  if (rc != kOK || Lo.kr == 0) return rc;
+  // C(I,J) += A(I, Kx) B(Kx, J) over depth kr: one product per C block, so
+  // the launch needs no ordering.
+  JobList JF;
+  Job f = j;
+  f.A = A;
+  f.B = B;
+  f.lda = lda;
+  f.ldb = ldb;
+  f.kr = Lo.kr;
+  f.nprod = P->Mx * P->Nx;
+  f.ngroups = f.nprod;
+  f.unit_base = 0;
+  f.order_base = -1;
+  f.flags = vec_flags(G, A, lda, B, ldb, Lo.ks, C, ldc);
+  f.prods = dp + P->R;
+  f.terms = dt;
+  JF.total_units = f.nprod * f.tiles_m * f.tiles_n;
+  JF.jobs.push_back(f);
+  return launch_jobs_dtype(dtype, JF, 1, nullptr, stream, true);
+}
+
+// One AB or (M-writing) Naive call C += A*B on `stream`: [pre-sums,] one
+// product launch (the products into M -- AB with its operand sums formed
+// while packing, Naive over S, T --, the k-fringe products into M, the right
+// and bottom fringe GEMMs straight into C), then M_Add into the C core.  Lo
+// must be the layout of this dtype (pack_layout with its element sizes and
+// naive_fused).
 inline int launch_pack(Plan* P, int pack, int dtype, const PackLayout& Lo, int m, int n, int k, const void* A,
                        long long lda, const void* B, long long ldb, void* C, long long ldc, void* ws,
                        hipStream_t stream) {
+  const bool naive = pack == kPackNaive;
+  if (naive && naive_fused(dtype))
+    return launch_naive_fused(P, dtype, Lo, m, n, k, A, lda, B, ldb, C, ldc, ws, stream);
   KGeo G;
   if (!dtype_geo(dtype, &G)) return kEUnsupported;
   Prod *dp, *xp, *ip;
   Term *dt, *xt, *it;
   int rc = device_tables(P, P->main, &dp, &dt);
-  if (rc == kOK) rc = device_tables(P, pack == kPackNaive ? P->diag : P->ab, &xp, &xt);
+  if (rc == kOK) rc = device_tables(P, naive ? P->diag : P->ab, &xp, &xt);
   if (rc == kOK) rc = device_tables(identity_plan(), &ip, &it);
   int *cptr, *cprod;
   double* cw;
@@ -759,67 +945,11 @@ inline int launch_pack(Plan* P, int pack, int dtype, const PackLayout& Lo, int m
   const void* pa = A;
   const void* pb = B;
   long long la = lda, lb = ldb, kblk = Lo.ks;
-  if (pack == kPackNaive) {
+  if (naive) {
     void* S = w + Lo.s_off;
     void* T = w + Lo.t_off;
-    const dim3 ga(ew_grid(Lo.ms, Lo.ks), P->R), gb(ew_grid(Lo.ks, Lo.ns), P->R);
-    if (dtype == kF64) {
-      SumJob sa{};
-      sa.X = static_cast<const double*>(A);
-      sa.Y = static_cast<double*>(S);
-      sa.ldx = lda;
-      sa.ldy = Lo.lds;
-      sa.rows = Lo.ms;
-      sa.cols = Lo.ks;
-      sa.ny = 1;
-      sa.side = 0;
-      sa.vec = ((uintptr_t)A & 15) == 0 && (lda & 1) == 0 && ((uintptr_t)S & 15) == 0;
-      sa.prods = dp;
-      sa.terms = dt;
-      presum_kernel<<<ga, dim3(kEwThreads), 0, stream>>>(sa);
-      if (hipGetLastError() != hipSuccess) return kEHip;
-      SumJob sb = sa;
-      sb.X = static_cast<const double*>(B);
-      sb.Y = static_cast<double*>(T);
-      sb.ldx = ldb;
-      sb.ldy = Lo.ldt;
-      sb.rows = Lo.ks;
-      sb.cols = Lo.ns;
-      sb.side = 1;
-      sb.vec = ((uintptr_t)B & 15) == 0 && (ldb & 1) == 0 && (Lo.ks & 1) == 0 && ((uintptr_t)T & 15) == 0;
-      presum_kernel<<<gb, dim3(kEwThreads), 0, stream>>>(sb);
-    } else {
-      SumJobL sa{};
-      sa.X = A;
-      sa.Y = S;
-      sa.ldx = lda;
-      sa.ldy = Lo.lds;
-      sa.rows = Lo.ms;
-      sa.cols = Lo.ks;
-      sa.ny = 1;
-      sa.side = 0;
-      sa.prods = dp;
-      sa.terms = dt;
-      SumJobL sb = sa;
-      sb.X = B;
-      sb.Y = T;
-      sb.ldx = ldb;
-      sb.ldy = Lo.ldt;
-      sb.rows = Lo.ks;
-      sb.cols = Lo.ns;
-      sb.side = 1;
-      if (dtype == kBF16) {
-        presum_lowp_kernel<kInBF16><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
-        presum_lowp_kernel<kInBF16><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
-      } else if (dtype == kF16) {
-        presum_lowp_kernel<kInF16><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
-        presum_lowp_kernel<kInF16><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
-      } else {
-        presum_lowp_kernel<kInF32><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
-        presum_lowp_kernel<kInF32><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
-      }
-    }
-    if (hipGetLastError() != hipSuccess) return kEHip;
+    rc = launch_presums(P, dtype, Lo, dp, dt, A, lda, B, ldb, S, T, stream);
+    if (rc != kOK) return rc;
     pa = S;
     la = Lo.lds;
     pb = T;
@@ -858,7 +988,7 @@ inline int launch_pack(Plan* P, int pack, int dtype, const PackLayout& Lo, int m
   };
   add_m_job(pa, la, pb, lb, kblk, 0, P->R, xp, xt);
   if (Lo.kr > 0) add_m_job(A, lda, B, ldb, Lo.ks, Lo.kr, P->Mx * P->Nx, xp + P->R, xt);
-  rc = launch_jobs_dtype(dtype, JL, pack == kPackNaive ? 1 : P->max_nnz, nullptr, stream, pack == kPackNaive);
+  rc = launch_jobs_dtype(dtype, JL, naive ? 1 : P->max_nnz, nullptr, stream, naive);
   if (rc != kOK) return rc;
   const dim3 gm(ew_grid(Lo.ms, Lo.ns), P->Mx * P->Nx);
   if (dtype == kF64) {

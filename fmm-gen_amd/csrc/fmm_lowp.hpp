@@ -646,17 +646,56 @@ __device__ __forceinline__ void l_contract(const Job& J, const Prod& P, const LG
 }
 
 // C_j = C_j + w_j * acc (C_j = w_j * acc for a store job) in FP32: lane l
-// holds rows 4*(l/16)+0..3 of column l%16 of each 16x16 block.
+// holds rows 4*(l/16)+0..3 of column l%16 of each 16x16 block.  In a tile
+// wholly inside the leaf block every read of a term is issued before its
+// first write (with a run-time ldc the compiler cannot prove two elements
+// distinct and would otherwise wait out each read-modify-write in turn).
 template <class C>
 __device__ __forceinline__ void l_scatter(const Job& J, const Prod& P, int row0, int col0, int lane, int wm, int wn,
                                           const f32x4 (&acc)[C::FM][C::FN]) {
   const int l16 = lane & 15, lk = lane >> 4;
   const bool store = (J.flags & kJobStore) != 0;
   const bool cvec = (J.flags & 4) != 0;
+  const bool inside = cvec && row0 + C::BM <= J.ms && col0 + C::BN <= J.ns;
   for (int t = 0; t < P.nc; ++t) {
     const Term tm = term_at(J, P.c0 + t);
     const float w = (float)tm.coef;
     gfloat* cb = (gfloat*)static_cast<float*>(J.C) + (long long)tm.bi * J.ms + (long long)tm.bj * J.ns * J.ldc;
+    if (inside) {
+      // RG block rows per round: all their reads, then all their writes
+      constexpr int RG = C::FM % 2 == 0 ? 2 : 1;
+#pragma unroll
+      for (int m0 = 0; m0 < C::FM; m0 += RG) {
+        f32x4 old[RG][C::FN];
+        if (!store) {
+#pragma unroll
+          for (int mi = 0; mi < RG; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < C::FN; ++ni)
+              old[mi][ni] = *(gf32x4*)(cb + row0 + wm * C::WTM + (m0 + mi) * 16 + 4 * lk +
+                                       (long long)(col0 + wn * C::WTN + ni * 16 + l16) * J.ldc);
+        }
+#pragma unroll
+        for (int mi = 0; mi < RG; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < C::FN; ++ni) {
+            const f32x4 v = acc[m0 + mi][ni];
+            f32x4 o;
+            if (store) {
+              o = f32x4{w * v.x, w * v.y, w * v.z, w * v.w};
+            } else {
+              const f32x4 x = old[mi][ni];
+              o.x = __builtin_fmaf(w, v.x, x.x);
+              o.y = __builtin_fmaf(w, v.y, x.y);
+              o.z = __builtin_fmaf(w, v.z, x.z);
+              o.w = __builtin_fmaf(w, v.w, x.w);
+            }
+            *(gf32x4*)(cb + row0 + wm * C::WTM + (m0 + mi) * 16 + 4 * lk +
+                       (long long)(col0 + wn * C::WTN + ni * 16 + l16) * J.ldc) = o;
+          }
+      }
+      continue;
+    }
 #pragma unroll
     for (int mi = 0; mi < C::FM; ++mi) {
       const int row = row0 + wm * C::WTM + mi * 16 + 4 * lk;

@@ -41,7 +41,7 @@ extern "C" {
  *               product formed as hi*hi + hi*lo + lo*hi on BF16 MFMA
  * The reference has FP64 only (common/include/bl_dgemm.h:52-130); the other
  * types have no counterpart there.  Every pack type takes every dtype; the
- * AB / NAIVE workspaces hold M in C's type and S, T in the input type. */
+ * AB workspace holds M in C's type, the NAIVE one S, T in the input type. */
 #define FMM_F64     0
 #define FMM_F32     1
 #define FMM_BF16    2
@@ -54,7 +54,9 @@ extern "C" {
  *          blocks directly;
  *   AB     operand sums formed while packing, product r into a workspace
  *          block M_r, then C_j += w_j M_r;
- *   NAIVE  S_r, T_r summed into workspaces, M_r = S_r T_r, then C_j += w_j M_r.
+ *   NAIVE  S_r, T_r summed into workspaces, M_r = S_r T_r, C_j += w_j M_r
+ *          (here applied from the product's accumulators, in product order,
+ *          instead of through a stored M_r).
  * The three apply the same C updates in the same order and agree bit for bit. */
 #define FMM_PACK_ABC    0
 #define FMM_PACK_AB     1
@@ -80,8 +82,8 @@ int fmm_plan_info(fmm_plan_t plan, int *Mx, int *Kx, int *Nx, int *rank);
  * FMM_F64 and is enough for every dtype (the others use larger C tiles). */
 size_t fmm_workspace_bytes(fmm_plan_t plan, int m, int n, int k);
 /* The same for any pack type: AB needs M (R [+ Mx*Nx] leaf blocks ms x ns),
- * NAIVE also S (R blocks ms x ks) and T (R blocks ks x ns).  Sized for
- * FMM_F64 elements, which bounds every other dtype. */
+ * NAIVE S (R blocks ms x ks), T (R blocks ks x ns) and product-order
+ * counters.  Sized for FMM_F64 elements, which bounds every other dtype. */
 size_t fmm_workspace_bytes_pack(fmm_plan_t plan, int pack, int m, int n, int k);
 
 /* ---- device-pointer entries (stream ordered, reentrant) -------------------

@@ -167,8 +167,9 @@ def test_lowp_argument_errors(torch, fmm, rt):
                          ids=["222L1", "333L1", "222x323"])
 def test_lowp_packs_equal_abc(torch, fmm, rt, oracle, spec, dtype, pack):
     """AB and Naive with low-precision inputs: the stored operand sums are
-    rounded like the ABC kernel's LDS sums and M_Add applies the same FP32
-    updates in the same order, so the results equal ABC's bit for bit."""
+    rounded like the ABC kernel's LDS sums, and AB's M_Add and Naive's fused
+    C updates (the k-fringe in a second launch) apply the same FP32 updates
+    in the same order, so the results equal ABC's bit for bit."""
     alg = fmm.build_algorithm(*spec)
     plan = rt.Plan(alg)
     Mx, Kx, Nx = alg.dims
