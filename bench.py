@@ -423,8 +423,8 @@ def main():
     abc_ms = kern_ms if best == "abc" else kernel_ms("abc")
     achieved = f_mfma / (kern_ms * 1e-3) / 1e12
     nnzU, nnzV, nnzW = alg.nnz()
-    if best == "naive":   # single-term products over S, T; every C block update applied once
-        b_min = 8 * (alg.rank * p.ms * p.ks + alg.rank * p.ks * p.ns) + 8 * 2 * nnzW * p.ms * p.ns
+    if best == "naive":   # FP64: single-term products over S, T; M blocks stored once
+        b_min = 8 * (alg.rank * p.ms * p.ks + alg.rank * p.ks * p.ns + alg.rank * p.ms * p.ns)
     elif best == "ab":
         b_min = 8 * (nnzU * p.ms * p.ks + nnzV * p.ks * p.ns + alg.rank * p.ms * p.ns)
     else:
