@@ -360,19 +360,6 @@ __device__ __forceinline__ void accumulate(const Srcs<CH>& S, bool first, const 
 template <int BK, int CH>
 __device__ __forceinline__ void finish_sums(const Job& J, const Prod& P, const Srcs<CH>& S0, int kb,
                                             const UnitGeo& g, Raw<BK, CH>& r, Slice<BK>& s) {
-  if constexpr (CH == 1) {
-    // Coefficients 1 (plain GEMMs, Naive products): the operands go to LDS as
-    // loaded.  1*x == x, so the result is the same; the FP64 multiplies this
-    // saves cost MFMA throughput (GEMM 9600^3: 59.9 -> 60.6 TFLOP/s).  (Storing
-    // r itself instead of copying it to s measured 53.7: worse registers.)
-    if (S0.u[0] == 1.0 && S0.v[0] == 1.0) {
-#pragma unroll
-      for (int e = 0; e < TileK<BK>::AE; ++e) s.a[e] = r.a[0][e];
-#pragma unroll
-      for (int e = 0; e < TileK<BK>::BE; ++e) s.b[e] = r.b[0][e];
-      return;
-    }
-  }
   accumulate<BK, CH>(S0, true, r, s);
   const int nmax = P.na > P.nb ? P.na : P.nb;
   for (int c = CH; c < nmax; c += CH) {
@@ -385,17 +372,41 @@ __device__ __forceinline__ void finish_sums(const Job& J, const Prod& P, const S
 
 template <int BK>
 __device__ __forceinline__ void store_slice(double (*As)[BM], double (*Bs)[TileK<BK>::BKP], const UnitGeo& g,
-                                            const Slice<BK>& s) {
+                                            const double (&sa)[TileK<BK>::AE], const double (&sb)[TileK<BK>::BE]) {
   constexpr int A_IT = TileK<BK>::A_IT, B_KPT = TileK<BK>::B_KPT;
 #pragma unroll
   for (int i = 0; i < A_IT; ++i) {
     double2 v;
-    v.x = s.a[2 * i];
-    v.y = s.a[2 * i + 1];
+    v.x = sa[2 * i];
+    v.y = sa[2 * i + 1];
     *reinterpret_cast<double2*>(&As[g.acol + A_CPP * i][g.arow]) = v;
   }
 #pragma unroll
-  for (int e = 0; e < B_KPT; ++e) Bs[g.bcol][g.bk + e] = s.b[e];
+  for (int e = 0; e < B_KPT; ++e) Bs[g.bcol][g.bk + e] = sb[e];
+}
+
+// Operand sums of slice kb (chunk 0 already loaded into r) stored to LDS.
+// The single-term kernel scales the loaded values in place, and only when
+// the coefficient is not 1 (plain GEMMs and Naive products have 1): the FP64
+// multiplies and register copies this saves cost MFMA throughput.
+template <int BK, int CH>
+__device__ __forceinline__ void pack_slice(const Job& J, const Prod& P, const Srcs<CH>& S, int kb, const UnitGeo& g,
+                                           Raw<BK, CH>& r, Slice<BK>& s, double (*As)[BM],
+                                           double (*Bs)[TileK<BK>::BKP]) {
+  if constexpr (CH == 1) {
+    if (S.u[0] != 1.0) {
+#pragma unroll
+      for (int e = 0; e < TileK<BK>::AE; ++e) r.a[0][e] = S.u[0] * r.a[0][e];
+    }
+    if (S.v[0] != 1.0) {
+#pragma unroll
+      for (int e = 0; e < TileK<BK>::BE; ++e) r.b[0][e] = S.v[0] * r.b[0][e];
+    }
+    store_slice<BK>(As, Bs, g, r.a[0], r.b[0]);
+  } else {
+    finish_sums<BK, CH>(J, P, S, kb, g, r, s);
+    store_slice<BK>(As, Bs, g, s.a, s.b);
+  }
 }
 
 // MFMAs of k-steps [KK0, KK1) of the slice.  v_mfma_f64_16x16x4f64 computes
@@ -590,8 +601,7 @@ __device__ __forceinline__ void sums_phase(const Job& J, const Prod& P, const Sr
                                            const UnitGeo& g, Raw<BK, CH>& r, Slice<BK>& s, double (*As)[BK][BM],
                                            double (*Bs)[BN][TileK<BK>::BKP], int nxt) {
   if (st + 1 < nsteps) {
-    finish_sums<BK, CH>(J, P, S, (st + 1) * BK, g, r, s);
-    store_slice<BK>(As[nxt], Bs[nxt], g, s);
+    pack_slice<BK, CH>(J, P, S, (st + 1) * BK, g, r, s, As[nxt], Bs[nxt]);
     if (st + 2 < nsteps) load_raw<BK, CH>(J, S, (st + 2) * BK, g, r);
   }
 }
@@ -607,8 +617,7 @@ __device__ __forceinline__ void contract(const Job& J, const Prod& P, const Unit
   Raw<BK, CH> r;
   Slice<BK> s;
   load_raw<BK, CH>(J, S, 0, g, r);
-  finish_sums<BK, CH>(J, P, S, 0, g, r, s);
-  store_slice<BK>(As[0], Bs[0], g, s);
+  pack_slice<BK, CH>(J, P, S, 0, g, r, s, As[0], Bs[0]);
   if (nsteps > 1) load_raw<BK, CH>(J, S, BK, g, r);
   __syncthreads();
   constexpr bool FP = F64Sched<CH>::frag_pipe, PR = F64Sched<CH>::setprio;
