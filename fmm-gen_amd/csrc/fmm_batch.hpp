@@ -59,10 +59,11 @@ struct Batch {
 
 inline int batch_reserve(Batch* b, size_t nj, size_t wsn);
 
-// Kernel variant a batch needs (0..3): FP64 by term count, low precision by
-// the C-tile shapes present.  A captured launch is valid for one variant.
+// Kernel variant a batch needs (0..3): FP64 by term count (2: single-term
+// products only, 0: up to 2 terms, 1: more), low precision by the C-tile
+// shapes present.  A captured launch is valid for one variant.
 inline int batch_variant(const Batch& b) {
-  if (b.dtype == kF64) return b.max_terms <= 2 ? 0 : 1;
+  if (b.dtype == kF64) return b.max_terms <= 1 ? 2 : b.max_terms <= 2 ? 0 : 1;
   if (b.has_t && b.has_d) return 3;
   return b.has_t ? 2 : 1;
 }
@@ -205,7 +206,11 @@ inline int batch_prepare(const Batch& b) {
     if (rc == kOK) rc = dev_info(reinterpret_cast<const void*>(k), nt, &di);
   };
   switch (b.dtype) {
-    case kF64: q(abc_f64_kernel<32, 2>, NT); q(abc_f64_kernel<16, 4>, NT); break;
+    case kF64:
+      q(abc_f64_kernel<32, 1>, NT);
+      q(abc_f64_kernel<32, 2>, NT);
+      q(abc_f64_kernel<16, 4>, NT);
+      break;
     case kF32:
       q(abc_lowp_kernel<CfgF32, CfgF32T>, CfgF32::NT);
       q(abc_lowp_kernel<CfgF32T, CfgF32T>, CfgF32::NT);
@@ -234,8 +239,11 @@ inline int batch_run(const Batch& b, hipStream_t s) {
   if (dev != b.dev) return kEInval;
   switch (b.dtype) {
     case kF64:
-      return batch_variant(b) == 0 ? launch_table(abc_f64_kernel<32, 2>, NT, b, s)
-                                   : launch_table(abc_f64_kernel<16, 4>, NT, b, s);
+      switch (batch_variant(b)) {
+        case 2: return launch_table(abc_f64_kernel<32, 1>, NT, b, s);
+        case 0: return launch_table(abc_f64_kernel<32, 2>, NT, b, s);
+        default: return launch_table(abc_f64_kernel<16, 4>, NT, b, s);
+      }
     case kF32: return batch_launch_lowp<CfgF32, CfgF32T>(b, s);
     case kBF16: return batch_launch_lowp<CfgBF16, CfgBF16T>(b, s);
     case kF16: return batch_launch_lowp<CfgF16, CfgF16T>(b, s);

@@ -375,7 +375,7 @@ class DeviceExecutor:
         need (capture synchronises the device, so it is done before serving)."""
         torch, rt = self.torch, self.rt
         for ln in lanes:
-            combos = [[("gemm", 0)], [("333L1", 0)]] if ln.dtype == "f64" else \
+            combos = [[("gemm", 0)], [("222L1", 0)], [("333L1", 0)]] if ln.dtype == "f64" else \
                 [[("gemm", 0)], [("gemm", 1)], [("gemm", 0), ("gemm", 1)]]
             for combo in combos:
                 reqs = []

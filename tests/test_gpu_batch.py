@@ -123,6 +123,42 @@ def test_batch_rerun_and_graph_replay(torch, fmm, rt, dtype):
         assert batch.status() == 0
 
 
+def test_f64_batch_variants(torch, fmm, rt):
+    """FP64 batches pick their kernel by the largest operand-sum term count:
+    only plain GEMMs -> the single-term kernel (variant 2), at most 2 terms
+    (222) -> variant 0, more (333) -> variant 1; each problem still equals its
+    single call bit for bit, and integer problems are exact."""
+    rng = np.random.default_rng(23)
+    p222 = rt.Plan(fmm.build_algorithm(["222"], [1]))
+    p333 = rt.Plan(fmm.build_algorithm(["333"], [1]))
+    for plan, want in ((None, 2), (p222, 0), (p333, 1)):
+        probs, singles, exact = [], [], []
+        for i, (m, n, k) in enumerate(shapes(rng, 8)):
+            A, B, C = int_mats(rng, m, n, k)
+            if i % 2:
+                A = rng.uniform(-1, 1, (m, k))
+                B = rng.uniform(-1, 1, (k, n))
+            dA, dB, dC = make(torch, rt, "f64", A, B, C)
+            probs.append((dA, dB, dC, plan, 0))
+            sA, sB, sC = make(torch, rt, "f64", A, B, C)
+            if plan is None:
+                rt.gemm(sA, sB, sC)
+            else:
+                plan.run(sA, sB, sC)
+            singles.append(sC)
+            exact.append(None if i % 2 else C + A @ B)
+        batch = rt.Batch(probs)
+        assert batch.variant() == want
+        batch.run()
+        torch.cuda.synchronize()
+        assert batch.status() == 0
+        for i, ((_, _, dC, _, _), sC, ex) in enumerate(zip(probs, singles, exact)):
+            got = dC.cpu().numpy()
+            assert np.array_equal(got, sC.cpu().numpy()), (want, i)
+            if ex is not None:
+                assert np.array_equal(got, ex), (want, i)
+
+
 def test_batch_errors_and_empty(torch, fmm, rt):
     A = rt.colmajor(8, 8, dtype=torch.float32)
     B = rt.colmajor(8, 8, dtype=torch.bfloat16)
