@@ -32,7 +32,6 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-
 namespace fmm {
 
 // One coefficient of a product: block (bi, bj) of the operand, scaled by coef.
@@ -303,6 +302,44 @@ __device__ __forceinline__ void load_raw(const Job& J, const Srcs<CH>& S, int kb
 #pragma unroll
         for (int e = 0; e < B_KPT; e += 2) {
           const f64x2 v = *(gcf64x2*)(base + e + g.boff);
+          r.b[t][e] = v.x;
+          r.b[t][e + 1] = v.y;
+        }
+      }
+    }
+    return;
+  }
+  if (CH == 1 && g.interior) {
+    // Last (partial) slice of an interior tile: every row and column is in
+    // range, only k runs out, so the loads stay 16-byte wide and only the
+    // k predicate remains (a leaf depth that is no multiple of BK, e.g. the
+    // 240-deep leaves of a 222 product at k = 480, ends every unit here).
+    // Single-term kernel only: in the operand-sum kernels the extra path
+    // cost registers (a spill) and measured slower.
+    const long long ka = (long long)kb * J.lda;
+    const int krem = g.depth - (kb + g.bk);
+#pragma unroll
+    for (int t = 0; t < CH; ++t) {
+      if (t < S.na) {
+        gcdouble* base = S.a[t] + ka;
+#pragma unroll
+        for (int i = 0; i < A_IT; ++i) {
+          f64x2 v = {0.0, 0.0};
+          if (g.acol + kb + A_CPP * i < g.depth) v = *(gcf64x2*)(base + (long long)(A_CPP * i) * J.lda + g.aoff);
+          r.a[t][2 * i] = v.x;
+          r.a[t][2 * i + 1] = v.y;
+        }
+      }
+      if (t < S.nb) {
+        gcdouble* base = S.b[t] + kb;
+#pragma unroll
+        for (int e = 0; e < B_KPT; e += 2) {
+          f64x2 v = {0.0, 0.0};
+          if (e + 1 < krem) {
+            v = *(gcf64x2*)(base + e + g.boff);
+          } else if (e < krem) {
+            v.x = base[e + g.boff];
+          }
           r.b[t][e] = v.x;
           r.b[t][e + 1] = v.y;
         }
