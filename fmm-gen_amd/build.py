@@ -21,9 +21,15 @@ ARCH = "gfx950"
 # amdgpu-mfma-vgpr-form: MFMA accumulators in VGPRs; with the default AGPR
 # form this compiler copies every accumulator AGPR<->VGPR around each loop
 # iteration of an MFMA chain, which costs more than the MFMAs themselves.
+# amdgpu-sched-strategy=max-memory-clause: the machine scheduler variant that
+# measured best for the FP64 single-term kernel (GEMM 9600^3 63.5 -> 65.8
+# TFLOP/s, Naive 333 L1 64.4 -> 66.3) and neutral for the others; the default
+# and the other strategies (max-ilp, iterative-ilp, iterative-minreg) were
+# timed beside it.
 HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-mllvm", "-amdgpu-mfma-vgpr-form=1"]
+            "-mllvm", "-amdgpu-mfma-vgpr-form=1",
+            "-mllvm", "-amdgpu-sched-strategy=max-memory-clause"]
 
 
 def hipcc() -> str:
