@@ -68,7 +68,7 @@ struct Job {
   const Term* terms;
 };
 
-constexpr int kInlineJobs = 4;
+constexpr int kInlineJobs = 8;
 constexpr int kJobStore = 8;
 // Low-precision kernels: run this job with the kernel's second C-tile shape
 // (128 x 256 instead of 256 x 128).
@@ -864,7 +864,8 @@ struct SumJob {
   int ny;              // blocks per block column of Y
   int side;            // 0: the A terms of each product, 1: the B terms
   int vec;             // 1: 16-byte accesses allowed (even rows/ld, aligned)
-  int pad;
+  int skip1;           // 1: leave out products with a single term on this side
+                       //    (their product jobs read the operand directly)
   const Prod* prods;
   const Term* terms;
 };
@@ -885,6 +886,7 @@ __global__ void __launch_bounds__(kEwThreads) presum_kernel(SumJob S) {
   const Prod* pp = S.prods + r;
   const int t0 = S.side ? pp->b0 : pp->a0;
   const int nt = S.side ? pp->nb : pp->na;
+  if (S.skip1 && nt == 1) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = r0 + 2 * lane;
   gdouble* y = gptr(S.Y) + (long long)(r % S.ny) * S.rows + (long long)(r / S.ny) * S.cols * S.ldy;

@@ -89,6 +89,10 @@ struct Plan {
   Table main;
   // Naive: product r reads workspace blocks S_r, T_r and writes M_r (FP64)...
   Table diag;
+  // ... the same products split by which operands have a single term and are
+  // read directly (bit 0: A, bit 1: B) instead of through S_r / T_r, in
+  // product order within each table (FP64 M-writing launch).
+  Table diag_by[4];
   // ... or updates its C blocks (the W terms of `main`) straight from the
   // accumulators, in product order per C tile, as ABC (other types).
   Table naive;
@@ -193,6 +197,10 @@ inline int build_plan(Plan* P, int nlev, const int* dims, int R, const double* U
   // `main` in a launch of their own.
   P->diag.prods.h.clear();
   P->diag.terms.h.clear();
+  for (Table& t : P->diag_by) {
+    t.prods.h.clear();
+    t.terms.h.clear();
+  }
   P->naive.prods.h.clear();
   P->naive.terms.h.clear();
   P->ab.prods.h.clear();
@@ -210,6 +218,23 @@ inline int build_plan(Plan* P, int nlev, const int* dims, int R, const double* U
     dt.push_back(Term{1.0, 0, r});
     d.nc = 1;
     P->diag.prods.h.push_back(d);
+
+    {
+      const Prod& o = prods[r];
+      const int combo = (o.na == 1 ? 1 : 0) | (o.nb == 1 ? 2 : 0);
+      std::vector<Term>& ct = P->diag_by[combo].terms.h;
+      Prod c{};
+      c.a0 = (int)ct.size();
+      ct.push_back(o.na == 1 ? terms[o.a0] : Term{1.0, 0, r});
+      c.na = 1;
+      c.b0 = (int)ct.size();
+      ct.push_back(o.nb == 1 ? terms[o.b0] : Term{1.0, 0, r});
+      c.nb = 1;
+      c.c0 = (int)ct.size();
+      ct.push_back(Term{1.0, 0, r});
+      c.nc = 1;
+      P->diag_by[combo].prods.h.push_back(c);
+    }
 
     std::vector<Term>& ft = P->naive.terms.h;
     Prod f{};
@@ -299,7 +324,8 @@ inline int device_clists(Plan* P, int** ptr, int** prod, double** w) {
 }
 
 inline void release_plan(Plan* P) {
-  for (Table* T : {&P->main, &P->diag, &P->naive, &P->ab}) {
+  for (Table* T : {&P->main, &P->diag, &P->diag_by[0], &P->diag_by[1], &P->diag_by[2], &P->diag_by[3], &P->naive,
+                   &P->ab}) {
     T->prods.release();
     T->terms.release();
   }
@@ -776,7 +802,7 @@ inline int ew_grid(int rows, int cols) { return tiles(rows, kEwRows) * tiles(col
 // workspace blocks (0, r) of S and T, with the ABC kernel's packing arithmetic.
 inline int launch_presums(const Plan* P, int dtype, const PackLayout& Lo, const Prod* dp, const Term* dt,
                           const void* A, long long lda, const void* B, long long ldb, void* S, void* T,
-                          hipStream_t stream) {
+                          hipStream_t stream, bool skip_single = false) {
   const dim3 ga(ew_grid(Lo.ms, Lo.ks), P->R), gb(ew_grid(Lo.ks, Lo.ns), P->R);
   if (dtype == kF64) {
     SumJob sa{};
@@ -789,6 +815,7 @@ inline int launch_presums(const Plan* P, int dtype, const PackLayout& Lo, const 
     sa.ny = 1;
     sa.side = 0;
     sa.vec = ((uintptr_t)A & 15) == 0 && (lda & 1) == 0 && ((uintptr_t)S & 15) == 0;
+    sa.skip1 = skip_single ? 1 : 0;
     sa.prods = dp;
     sa.terms = dt;
     presum_kernel<<<ga, dim3(kEwThreads), 0, stream>>>(sa);
@@ -945,10 +972,14 @@ inline int launch_pack(Plan* P, int pack, int dtype, const PackLayout& Lo, int m
   const void* pa = A;
   const void* pb = B;
   long long la = lda, lb = ldb, kblk = Lo.ks;
+  // FP64 Naive: products with a single A (B) term read that block of A (B)
+  // directly (scaled by its coefficient in the kernel, as the pre-sum would
+  // scale it), so their pre-sums are skipped; one job per combination.
+  const bool split = naive && dtype == kF64;
   if (naive) {
     void* S = w + Lo.s_off;
     void* T = w + Lo.t_off;
-    rc = launch_presums(P, dtype, Lo, dp, dt, A, lda, B, ldb, S, T, stream);
+    rc = launch_presums(P, dtype, Lo, dp, dt, A, lda, B, ldb, S, T, stream, split);
     if (rc != kOK) return rc;
     pa = S;
     la = Lo.lds;
@@ -986,7 +1017,20 @@ inline int launch_pack(Plan* P, int pack, int dtype, const PackLayout& Lo, int m
     JL.total_units += nprod * j.tiles_m * j.tiles_n;
     JL.jobs.push_back(j);
   };
-  add_m_job(pa, la, pb, lb, kblk, 0, P->R, xp, xt);
+  if (split) {
+    for (int c = 0; c < 4; ++c) {
+      const int n = (int)P->diag_by[c].prods.h.size();
+      if (n == 0) continue;
+      Prod* cp;
+      Term* ct;
+      rc = device_tables(P, P->diag_by[c], &cp, &ct);
+      if (rc != kOK) return rc;
+      const bool da = (c & 1) != 0, db = (c & 2) != 0;
+      add_m_job(da ? A : pa, da ? lda : la, db ? B : pb, db ? ldb : lb, db ? (long long)Lo.ks : 0, 0, n, cp, ct);
+    }
+  } else {
+    add_m_job(pa, la, pb, lb, kblk, 0, P->R, xp, xt);
+  }
   if (Lo.kr > 0) add_m_job(A, lda, B, ldb, Lo.ks, Lo.kr, P->Mx * P->Nx, xp + P->R, xt);
   rc = launch_jobs_dtype(dtype, JL, naive ? 1 : P->max_nnz, nullptr, stream, naive);
   if (rc != kOK) return rc;
