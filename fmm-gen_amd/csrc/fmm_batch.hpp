@@ -43,17 +43,26 @@ struct Batch {
   size_t jobs_cap = 0, ws_cap = 0;
   char* staging = nullptr;
   size_t staging_cap = 0;
+  // host-mapped pinned word: every run ends by copying its error flags here
+  // (err_copy_kernel), so a caller that saw the run complete (an event) reads
+  // them without synchronising the device
+  int* status_h = nullptr;
+  int* status_d = nullptr;
   int generation = 0;   // bumped whenever a device buffer is reallocated
+  bool valid = false;   // false after a failed batch_set: runs are refused
   void release() {
     if (djobs) (void)hipFree(djobs);
     if (dhdr) (void)hipFree(dhdr);
     if (dws) (void)hipFree(dws);
     if (staging) (void)hipHostFree(staging);
+    if (status_h) (void)hipHostFree(status_h);
     djobs = nullptr;
     dhdr = nullptr;
     dws = nullptr;
     staging = nullptr;
+    status_h = status_d = nullptr;
     jobs_cap = ws_cap = staging_cap = 0;
+    valid = false;
   }
 };
 
@@ -78,11 +87,13 @@ inline int batch_set(Batch* b, int dtype, int nspec, const BatchSpec* specs, hip
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return kEHip;
   if (b->dev >= 0 && b->dev != dev) return kEInval;
-  b->dev = dev;
-  b->dtype = dtype;
-  b->JL = JobList();
-  b->max_terms = 1;
-  b->has_t = b->has_d = false;
+  // The new contents are built in locals and committed only once the upload
+  // is enqueued; on any failure the batch is marked invalid, so no run can
+  // pair the old device table with the new kernel variant or vice versa.
+  b->valid = false;
+  JobList JL;
+  int max_terms = 1;
+  bool has_t = false, has_d = false;
   Prod* ip;
   Term* it;
   int rc = device_tables(identity_plan(), &ip, &it);
@@ -90,30 +101,34 @@ inline int batch_set(Batch* b, int dtype, int nspec, const BatchSpec* specs, hip
   for (int i = 0; i < nspec; ++i) {
     const BatchSpec& s = specs[i];
     if (s.m < 0 || s.n < 0 || s.k < 0 || s.lda < std::max(1, s.m) || s.ldb < std::max(1, s.k) ||
-        s.ldc < std::max(1, s.m))
+        s.ldc < std::max(1, s.m) || (s.tile != 0 && s.tile != 1))
       return kEInval;
+  }
+  for (int i = 0; i < nspec; ++i) {
+    const BatchSpec& s = specs[i];
     if (s.m == 0 || s.n == 0 || s.k == 0) continue;
     KGeo G;
     int tflag = 0;
     dtype_geo(dtype, &G, s.tile, &tflag);
-    const size_t first = b->JL.jobs.size();
+    const size_t first = JL.jobs.size();
     if (s.plan) {
       Prod* dp;
       Term* dt;
       if ((rc = device_tables(s.plan, &dp, &dt)) != kOK) return rc;
-      add_fmm_jobs(b->JL, G, *s.plan, dp, dt, ip, it, s.m, s.n, s.k, s.A, s.lda, s.B, s.ldb, s.C, s.ldc);
-      b->max_terms = std::max(b->max_terms, s.plan->max_nnz);
+      add_fmm_jobs(JL, G, *s.plan, dp, dt, ip, it, s.m, s.n, s.k, s.A, s.lda, s.B, s.ldb, s.C, s.ldc);
+      max_terms = std::max(max_terms, s.plan->max_nnz);
     } else {
-      add_gemm_job(b->JL, G, ip, it, s.m, s.n, s.k, s.A, s.lda, s.B, s.ldb, s.C, s.ldc);
+      add_gemm_job(JL, G, ip, it, s.m, s.n, s.k, s.A, s.lda, s.B, s.ldb, s.C, s.ldc);
     }
-    for (size_t j = first; j < b->JL.jobs.size(); ++j) {
-      b->JL.jobs[j].flags |= tflag;
-      (tflag ? b->has_t : b->has_d) = true;
+    for (size_t j = first; j < JL.jobs.size(); ++j) {
+      JL.jobs[j].flags |= tflag;
+      (tflag ? has_t : has_d) = true;
     }
   }
-  group_products(b->JL, 1);
-  const size_t nj = b->JL.jobs.size();
-  const size_t wsn = (size_t)kWsCounters + (size_t)b->JL.counters;
+  group_products(JL, 1);
+  const size_t nj = JL.jobs.size();
+  const size_t wsn = (size_t)kWsCounters + (size_t)JL.counters;
+  b->dev = dev;
   int rc2 = batch_reserve(b, nj, wsn);
   if (rc2 != kOK) return rc2;
   const size_t bytes = sizeof(Job) * nj + 2 * sizeof(int);
@@ -126,18 +141,36 @@ inline int batch_set(Batch* b, int dtype, int nspec, const BatchSpec* specs, hip
       return kENoMem;
     }
   }
-  int hdr[2] = {(int)nj, b->JL.total_units};
-  if (nj) memcpy(b->staging, b->JL.jobs.data(), sizeof(Job) * nj);
+  int hdr[2] = {(int)nj, JL.total_units};
+  if (nj) memcpy(b->staging, JL.jobs.data(), sizeof(Job) * nj);
   memcpy(b->staging + sizeof(Job) * nj, hdr, sizeof(hdr));
   if (nj && hipMemcpyAsync(b->djobs, b->staging, sizeof(Job) * nj, hipMemcpyHostToDevice, stream) != hipSuccess)
     return kEHip;
   if (hipMemcpyAsync(b->dhdr, b->staging + sizeof(Job) * nj, sizeof(hdr), hipMemcpyHostToDevice, stream) != hipSuccess)
     return kEHip;
+  b->dtype = dtype;
+  b->JL = std::move(JL);
+  b->max_terms = max_terms;
+  b->has_t = has_t;
+  b->has_d = has_d;
+  b->valid = true;
   return kOK;
 }
 
 // Grow the device buffers to hold `nj` jobs and `wsn` workspace ints.
 inline int batch_reserve(Batch* b, size_t nj, size_t wsn) {
+  if (!b->status_h) {
+    if (hipHostMalloc(&b->status_h, 64, hipHostMallocMapped) != hipSuccess) {
+      b->status_h = nullptr;
+      return kENoMem;
+    }
+    *b->status_h = 0;
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&b->status_d), b->status_h, 0) != hipSuccess) {
+      (void)hipHostFree(b->status_h);
+      b->status_h = b->status_d = nullptr;
+      return kEHip;
+    }
+  }
   if (!b->dhdr) {
     if (hipMalloc(&b->dhdr, 2 * sizeof(int)) != hipSuccess) return kENoMem;
     ++b->generation;
@@ -178,11 +211,13 @@ inline int launch_table(void (*kern)(Launch), int nt, const Batch& b, hipStream_
   L.jobs = b.djobs;
   L.hdr = b.dhdr;
   L.ws = b.dws;
-  L.spin_limit = di.spin_ticks;
+  L.spin_limit = spin_ticks(di);
   const int n = (int)b.ws_cap;
   hipLaunchKernelGGL(ws_clear_kernel, dim3(std::min((n + 255) / 256, 1024)), dim3(256), 0, stream, b.dws, n);
   if (hipGetLastError() != hipSuccess) return kEHip;
   hipLaunchKernelGGL(kern, dim3(std::max(1, di.cus * di.blocks_per_cu)), dim3(nt), 0, stream, L);
+  if (hipGetLastError() != hipSuccess) return kEHip;
+  hipLaunchKernelGGL(err_copy_kernel, dim3(1), dim3(64), 0, stream, b.status_d, b.dws);
   if (hipGetLastError() != hipSuccess) return kEHip;
   return kOK;
 }
@@ -233,7 +268,7 @@ inline int batch_prepare(const Batch& b) {
 }
 
 inline int batch_run(const Batch& b, hipStream_t s) {
-  if (!b.djobs || !b.dws || !b.dhdr) return kEInval;
+  if (!b.valid || !b.djobs || !b.dws || !b.dhdr || !b.status_d) return kEInval;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return kEHip;
   if (dev != b.dev) return kEInval;

@@ -26,8 +26,11 @@ inline bool bad_dims(int m, int n, int k, int lda, int ldb, int ldc) {
   return false;
 }
 
+// err_dst: where the flags of this call's ordered launch are OR-ed (a word
+// of the caller's status header; nullptr: they stay in ws, or go to the
+// device's sticky word when the call allocates ws itself).
 inline int run_abc(Plan* P, int dtype, int m, int n, int k, const void* A, int lda, const void* B, int ldb,
-                   void* C, int ldc, void* ws, size_t ws_bytes, hipStream_t s) {
+                   void* C, int ldc, void* ws, size_t ws_bytes, hipStream_t s, int* err_dst = nullptr) {
   if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
   KGeo G;
   if (!dtype_geo(dtype, &G)) return kEUnsupported;
@@ -52,6 +55,10 @@ inline int run_abc(Plan* P, int dtype, int m, int n, int k, const void* A, int l
     return kEInval;
   }
   rc = launch_jobs_dtype(dtype, JL, P->max_nnz, w, s, P->unit_single);
+  if (rc == kOK && need > 0) {
+    if (err_dst) rc = fold_error(err_dst, w, s);
+    else if (own) rc = fold_owned(w, s);
+  }
   if (own) (void)hipFreeAsync(w, s);
   return rc;
 }
@@ -77,7 +84,10 @@ inline int run_fmm(Plan* P, int pack, int dtype, int m, int n, int k, const void
     return kEInval;
   }
   int rc = launch_pack(P, pack, dtype, Lo, m, n, k, A, lda, B, ldb, C, ldc, w, s);
-  if (own) (void)hipFreeAsync(w, s);
+  if (own) {
+    if (rc == kOK) rc = fold_owned(static_cast<const int*>(w) + Lo.o_off / sizeof(int), s);
+    (void)hipFreeAsync(w, s);
+  }
   return rc;
 }
 
@@ -166,6 +176,16 @@ inline int host_call(int m, int n, int k, const double* A, int lda, const double
   return kOK;
 }
 
+// End of a host-pointer call: wait for the device and turn the flags its
+// library-owned workspaces left into the call's status.
+inline int host_finish(int rc) {
+  if (rc != kOK) return rc;
+  int flags = 0;
+  rc = take_sticky(&flags);
+  if (rc != kOK) return rc;
+  return flags ? kESpin : kOK;
+}
+
 inline void report(const char* fn, int rc) {
   if (rc != kOK) fprintf(stderr, "%s(): error %d\n", fn, rc);
 }
@@ -243,7 +263,12 @@ int fmm_gemm(int dtype, int m, int n, int k, const void* A, int lda, const void*
 }
 
 int fmm_ws_status(const void* ws, int* flags) {
-  if (!ws || !flags) return fmm::kEInval;
+  if (!flags) return fmm::kEInval;
+  if (!ws) {
+    const int rc = fmm::take_sticky(flags);
+    if (rc != fmm::kOK) return rc;
+    return *flags ? fmm::kESpin : fmm::kOK;
+  }
   if (hipDeviceSynchronize() != hipSuccess) return fmm::kEHip;
   int v = 0;
   if (hipMemcpy(&v, static_cast<const int*>(ws) + fmm::kWsError, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
@@ -258,8 +283,7 @@ int fmm_host_run(fmm_plan_t plan, int pack, int m, int n, int k, const double* A
   return fmm::host_call(m, n, k, A, lda, B, ldb, C, ldc,
                         [&](const double* a, int la, const double* b, int lb, double* c, int lc) {
                           int rc = fmm::run_fmm(&plan->p, pack, fmm::kF64, m, n, k, a, la, b, lb, c, lc, nullptr, 0, nullptr);
-                          if (rc == fmm::kOK && hipDeviceSynchronize() != hipSuccess) rc = fmm::kEHip;
-                          return rc;
+                          return fmm::host_finish(rc);
                         });
 }
 
@@ -386,6 +410,12 @@ int fmm_batch_status(fmm_batch_t batch, int* flags) {
   return fmm_ws_status(batch->b.dws, flags);
 }
 
+int fmm_batch_last_status(fmm_batch_t batch, int* flags) {
+  if (!batch || !flags || !batch->b.status_h) return fmm::kEInval;
+  *flags = __atomic_load_n(batch->b.status_h, __ATOMIC_ACQUIRE);
+  return *flags ? fmm::kESpin : fmm::kOK;
+}
+
 int fmm_batch_destroy(fmm_batch_t batch) {
   if (!batch) return fmm::kEInval;
   int cur = 0;
@@ -441,12 +471,17 @@ int fmm_refine(fmm_plan_t main_plan, int main_dtype, fmm_plan_t corr_plan, int c
   } else if (ws_bytes < need) {
     return kEInval;
   }
+  int rc0 = kOK;
+  // status header: the error word the sub-calls' ordered launches fold into
+  int* hdr = reinterpret_cast<int*>(w + Lo.hdr);
+  hipLaunchKernelGGL(ws_clear_kernel, dim3(1), dim3(256), 0, s, hdr, kWsCounters);
+  if (hipGetLastError() != hipSuccess) rc0 = kEHip;
   void* AA = w + Lo.aa;
   void* BB = w + Lo.bb;
   float* Ct = reinterpret_cast<float*>(w + Lo.ct);
   float* Et = reinterpret_cast<float*>(w + Lo.et);
   const long long mk = (long long)m * k, kn = (long long)k * n, mn = (long long)Lo.lt * n;
-  int rc = kOK;
+  int rc = rc0;
   auto chk = [&]() {
     if (rc == kOK && hipGetLastError() != hipSuccess) rc = kEHip;
   };
@@ -478,17 +513,20 @@ int fmm_refine(fmm_plan_t main_plan, int main_dtype, fmm_plan_t corr_plan, int c
   chk();
   if (rc == kOK)
     rc = main_plan ? run_abc(&main_plan->p, main_dtype, m, n, k, w + Lo.a1, (int)Lo.la1, w + Lo.b1, (int)Lo.lb1, Ct,
-                             (int)Lo.lt, w + Lo.wsm, Lo.wsc - Lo.wsm, s)
+                             (int)Lo.lt, w + Lo.wsm, Lo.wsc - Lo.wsm, s, hdr + kWsError)
                    : run_gemm(main_dtype, m, n, k, w + Lo.a1, (int)Lo.la1, w + Lo.b1, (int)Lo.lb1, Ct, (int)Lo.lt, s);
   if (rc == kOK)
     rc = corr_plan ? run_abc(&corr_plan->p, corr_dtype, m, n, 2 * k, AA, (int)Lo.laa, BB, (int)Lo.lbb, Et,
-                             (int)Lo.lt, w + Lo.wsc, Lo.bytes - Lo.wsc, s)
+                             (int)Lo.lt, w + Lo.wsc, Lo.bytes - Lo.wsc, s, hdr + kWsError)
                    : run_gemm(corr_dtype, m, n, 2 * k, AA, (int)Lo.laa, BB, (int)Lo.lbb, Et, (int)Lo.lt, s);
   if (rc == kOK) {
     combine_kernel<<<ew_blocks((long long)m * n), kRefThreads, 0, s>>>(m, n, Ct, Et, Lo.lt, C, ldc);
     chk();
   }
-  if (own) (void)hipFreeAsync(w, s);
+  if (own) {
+    if (rc == kOK) rc = fold_owned(hdr, s);
+    (void)hipFreeAsync(w, s);
+  }
   return rc;
 }
 

@@ -1003,4 +1003,22 @@ __global__ void ws_clear_kernel(int* ws, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ws[i] = 0;
 }
 
+// dst |= error word of workspace ws (one thread): carries the flags of a
+// launch whose workspace is about to be released, or of a sub-call, to the
+// word the caller reads.
+__global__ void err_fold_kernel(int* dst, const int* ws) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int v = __hip_atomic_load(ws + kWsError, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v != 0) atomicOr(dst, v);
+  }
+}
+
+// *dst = error word of ws (one thread); dst may be host-mapped pinned memory,
+// read by the host once the launch is known to be complete.
+__global__ void err_copy_kernel(int* dst, const int* ws) {
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    __hip_atomic_store(dst, __hip_atomic_load(ws + kWsError, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 }  // namespace fmm

@@ -350,9 +350,22 @@ inline Plan* identity_plan() {
 struct DevInfo {
   int cus = 0;
   int blocks_per_cu = 0;
-  long long spin_ticks = 0;
+  long long spin_ticks = 0;       // order-wait bound (FMM_SPIN_LIMIT_MS, default 5 s)
+  long long ticks_per_ms = 0;
   bool xcd_round_robin = false;   // workgroup i runs on XCC i % 8 (probed)
 };
+
+// Bound of one product-order wait, in wall_clock64() ticks: FMM_SPIN_LIMIT_MS
+// milliseconds when that variable is set (0 flags every wait that is not
+// satisfied at once; used to exercise the error path), else 5 s.
+inline long long spin_ticks(const DevInfo& d) {
+  const char* e = getenv("FMM_SPIN_LIMIT_MS");
+  if (e && *e) {
+    const long long ms = atoll(e);
+    if (ms >= 0) return ms * d.ticks_per_ms;
+  }
+  return d.spin_ticks;
+}
 
 // One-time probe: launch 256 workgroups and check that workgroup i reports
 // XCC id i % 8 (the dispatch pattern XCD-local hand-over relies on).
@@ -403,9 +416,67 @@ inline int dev_info(const void* kernel, int nt, DevInfo* out) {
   e.d.cus = cus;
   e.d.blocks_per_cu = std::max(nb, 1);
   e.d.xcd_round_robin = rr[dev] == 1;
+  e.d.ticks_per_ms = (long long)khz;
   e.d.spin_ticks = (long long)khz * 1000LL * 5LL;   // 5 s
   cache.push_back(e);
   *out = e.d;
+  return kOK;
+}
+
+// Per-device error word of the calls whose workspace the library allocated
+// itself (ws == NULL): before such a workspace is released, the flags its
+// launches left are OR-ed into this word (err_fold_kernel, stream ordered),
+// and fmm_ws_status(NULL, ...) reads and clears it.
+struct StickyErrors {
+  std::mutex mu;
+  std::vector<int*> word;
+};
+inline StickyErrors& sticky_errors() {
+  static StickyErrors s;
+  return s;
+}
+
+inline int sticky_word(int** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return kEHip;
+  StickyErrors& s = sticky_errors();
+  std::lock_guard<std::mutex> g(s.mu);
+  if ((int)s.word.size() <= dev) s.word.resize(dev + 1, nullptr);
+  if (!s.word[dev]) {
+    int* p = nullptr;
+    if (hipMalloc(&p, 256) != hipSuccess) return kENoMem;
+    if (hipMemset(p, 0, 256) != hipSuccess) {
+      (void)hipFree(p);
+      return kEHip;
+    }
+    s.word[dev] = p;
+  }
+  *out = s.word[dev];
+  return kOK;
+}
+
+// dst |= the error word of workspace ws, on `stream`.
+inline int fold_error(int* dst, const int* ws, hipStream_t stream) {
+  hipLaunchKernelGGL(err_fold_kernel, dim3(1), dim3(64), 0, stream, dst, ws);
+  return hipGetLastError() == hipSuccess ? kOK : kEHip;
+}
+
+// Fold the flags of a library-owned workspace into the device's sticky word.
+inline int fold_owned(const int* ws, hipStream_t stream) {
+  int* w = nullptr;
+  int rc = sticky_word(&w);
+  return rc == kOK ? fold_error(w, ws, stream) : rc;
+}
+
+// Read and clear the sticky word of the current device (synchronises it).
+inline int take_sticky(int* flags) {
+  *flags = 0;
+  int* w = nullptr;
+  int rc = sticky_word(&w);
+  if (rc != kOK) return rc;
+  if (hipDeviceSynchronize() != hipSuccess) return kEHip;
+  if (hipMemcpy(flags, w, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return kEHip;
+  if (*flags != 0 && hipMemset(w, 0, sizeof(int)) != hipSuccess) return kEHip;
   return kOK;
 }
 
@@ -612,7 +683,7 @@ inline int launch_jobs_with(void (*kern)(Launch), int nt, JobList JL, int* ws, h
   L.use_ticket = JL.ordered ? 1 : 0;
   L.jobs = nullptr;
   L.ws = ws;
-  L.spin_limit = di.spin_ticks;
+  L.spin_limit = spin_ticks(di);
   for (size_t i = 0; i < JL.jobs.size(); ++i) L.inl[i] = JL.jobs[i];
   if (JL.ordered) {
     if (!ws) return kEInval;
@@ -744,7 +815,9 @@ struct PackLayout {
   int nlive = 0;                       // AB: M blocks written (R [+ Mx*Nx])
   long long ldm = 0, lds = 0, ldt = 0; // leading dimensions of M, S, T
   size_t m_off = 0, s_off = 0, t_off = 0, o_off = 0, bytes = 0;
-  int o_ints = 0;                      // Naive: ints of the ordered launch's ws
+  int o_ints = 0;                      // ints of the status header at offset 0: the
+                                       // ordered launch's ws (fused Naive), else
+                                       // the error word area alone
 };
 
 inline size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -774,20 +847,21 @@ inline bool pack_layout(const Plan& P, int pack, int m, int n, int k, PackLayout
   o.ldm = o.ms;                        // ms % 8 == 0: 16-byte columns
   o.lds = o.ms;
   o.ldt = (o.ks + 7) & ~7;             // T columns stay 16-byte aligned for every type
-  size_t off = 0;
+  // The workspace starts with a status header whose error word sits where
+  // an ABC workspace has it (fmm_ws_status reads it for every pack type):
+  // for the fused Naive launch the whole ordered workspace (order counters,
+  // one per C tile of the smallest tile shape, 128 x 128, of any type), else
+  // the error word area alone, cleared by the call.
+  o.o_off = 0;
+  o.o_ints = (pack == kPackNaive && fused) ? kWsCounters + tiles(o.ms, 128) * tiles(o.ns, 128) : kWsCounters;
+  size_t off = round256(sizeof(int) * (size_t)o.o_ints);
   if (pack == kPackNaive) {
     o.s_off = off;
     off += round256((size_t)sin * (size_t)o.lds * o.ks * P.R);
     o.t_off = off;
     off += round256((size_t)sin * (size_t)o.ldt * o.ns * P.R);
   }
-  if (pack == kPackNaive && fused) {
-    // order counters of the product launch, one per C tile of the smallest
-    // tile shape (128 x 128) of any type
-    o.o_off = off;
-    o.o_ints = kWsCounters + tiles(o.ms, 128) * tiles(o.ns, 128);
-    off += round256(sizeof(int) * (size_t)o.o_ints);
-  } else {
+  if (!(pack == kPackNaive && fused)) {
     o.m_off = off;
     off += round256((size_t)sc * (size_t)o.ldm * o.ns * o.nlive);
   }
@@ -968,6 +1042,10 @@ inline int launch_pack(Plan* P, int pack, int dtype, const PackLayout& Lo, int m
   if (rc == kOK) rc = device_clists(P, &cptr, &cprod, &cw);
   if (rc != kOK) return rc;
   char* w = static_cast<char*>(ws);
+  // no ordered launch in this call: its status header only needs a clear error word
+  hipLaunchKernelGGL(ws_clear_kernel, dim3(1), dim3(256), 0, stream, reinterpret_cast<int*>(w + Lo.o_off),
+                     Lo.o_ints);
+  if (hipGetLastError() != hipSuccess) return kEHip;
   void* M = w + Lo.m_off;
   const void* pa = A;
   const void* pb = B;

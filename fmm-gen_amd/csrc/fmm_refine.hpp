@@ -96,7 +96,7 @@ inline long long round8(long long x) { return (x + 7) & ~7LL; }
 // Workspace layout of one refined product.
 struct RefLayout {
   long long la1, lb1, laa, lbb, lt;
-  size_t a1, b1, aa, bb, ct, et, wsm, wsc, bytes;
+  size_t hdr, a1, b1, aa, bb, ct, et, wsm, wsc, bytes;
 };
 
 inline int in_size(int dtype) { return (dtype == kBF16 || dtype == kF16) ? 2 : 4; }
@@ -116,6 +116,9 @@ inline RefLayout ref_layout(int main_dtype, int corr_dtype, int m, int n, int k,
     off += round256(std::max<size_t>(bytes, 1));
     return o;
   };
+  // status header at offset 0: its error word (where an ABC workspace has
+  // it) collects the flags of both sub-calls
+  L.hdr = take(sizeof(int) * (size_t)kWsCounters);
   L.a1 = take((size_t)L.la1 * k * s);
   L.b1 = take((size_t)L.lb1 * n * s);
   L.aa = take((size_t)L.laa * 2 * k * sc);

@@ -33,8 +33,8 @@ EXPORTS = ("fmm_plan_create", "fmm_plan_destroy", "fmm_plan_info", "fmm_workspac
            "fmm_ws_status", "fmm_host_abc", "fmm_host_run", "bl_dgemm", "bl_dgemm_ref",
            "bl_finalize", "fmm_batch_create", "fmm_batch_set", "fmm_batch_run", "fmm_batch_variant",
            "fmm_batch_info", "fmm_batch_status", "fmm_batch_destroy", "fmm_batch_reserve",
-           "fmm_batch_generation", "fmm_refine", "fmm_refine_workspace_bytes", "fmm_timing",
-           "fmm_timing_last_ms")
+           "fmm_batch_generation", "fmm_batch_last_status", "fmm_refine", "fmm_refine_workspace_bytes",
+           "fmm_timing", "fmm_timing_last_ms")
 
 
 class FmmError(RuntimeError):
@@ -251,10 +251,15 @@ def gemm(A, B, C, stream=None, library=None, split: bool = False) -> None:
                      _ld(C), _stream_ptr(stream)), f"fmm_gemm({DTYPE_NAMES[dt]})")
 
 
-def ws_status(ws, library=None) -> int:
+def ws_status(ws=None, library=None) -> int:
+    """Error flags of the last call that used workspace ``ws``; with
+    ws=None, of every call since the last query that allocated its own
+    workspace (cleared by the query).  Synchronises the device."""
     L = library or lib()
     flags = ct.c_int(0)
-    L.fmm_ws_status(ws.data_ptr(), ct.byref(flags))
+    rc = L.fmm_ws_status(None if ws is None else ws.data_ptr(), ct.byref(flags))
+    if rc not in (0, -5):
+        check(rc, "fmm_ws_status")
     return int(flags.value)
 
 
@@ -286,6 +291,8 @@ def _declare_batch(L):
     L.fmm_batch_info.restype = i
     L.fmm_batch_status.argtypes = [vp, ct.POINTER(i)]
     L.fmm_batch_status.restype = i
+    L.fmm_batch_last_status.argtypes = [vp, ct.POINTER(i)]
+    L.fmm_batch_last_status.restype = i
     L.fmm_batch_destroy.argtypes = [vp]
     L.fmm_batch_destroy.restype = i
     L._batch_declared = True
@@ -369,8 +376,16 @@ class Batch:
         check(self._lib.fmm_batch_run(self.handle, _stream_ptr(stream)), "fmm_batch_run")
 
     def status(self) -> int:
+        """Error flags of the last run (synchronises the device)."""
         flags = ct.c_int(0)
         self._lib.fmm_batch_status(self.handle, ct.byref(flags))
+        return int(flags.value)
+
+    def last_status(self) -> int:
+        """Error flags of the last completed run, without synchronising
+        (valid once the caller has seen that run finish, e.g. by an event)."""
+        flags = ct.c_int(0)
+        self._lib.fmm_batch_last_status(self.handle, ct.byref(flags))
         return int(flags.value)
 
     def close(self):

@@ -58,6 +58,7 @@ class Request:
     t_done: Optional[float] = None
     dev: Optional[tuple] = None     # device (A, B, C) while resident
     result: Optional[np.ndarray] = None
+    error: int = 0                  # error flags of the launch that ran it (0: result valid)
 
     def __post_init__(self):
         self.nbytes = request_bytes(self.m, self.n, self.k, self.dtype)
@@ -109,8 +110,10 @@ class Server:
         n = 0
         for ln in self.lanes:
             if ln.busy and self.ex.finished(ln):
+                err = self.ex.status(ln)
                 for r in ln.reqs:
                     r.t_done = now
+                    r.error = err
                     self.ex.collect(r)
                     self.ex.free(r)
                     self.in_use -= r.nbytes
@@ -198,6 +201,7 @@ class Server:
             "launches": self.launches,
             "mean_batch": round(float(np.mean(self.batch_sizes)), 2) if self.batch_sizes else 0.0,
             "peak_hbm_gb": round(self.peak_in_use / 1e9, 3),
+            "failed": sum(1 for r in self.done if r.error),
         }
 
 
@@ -254,6 +258,9 @@ class SimExecutor:
 
     def finished(self, ln):
         return self.clock() >= ln.state["end"]
+
+    def status(self, ln):
+        return ln.state.get("error", 0)
 
     def collect(self, r):
         if r.host is not None:
@@ -359,6 +366,11 @@ class DeviceExecutor:
 
     def finished(self, ln):
         return ln.state["event"].query()
+
+    def status(self, ln):
+        """Error flags of the lane's completed launch (the batch copies them
+        to host-mapped memory at the end of every run, so no sync)."""
+        return ln.state["batch"].last_status()
 
     def collect(self, r):
         if self.keep_results:

@@ -116,7 +116,15 @@ int fmm_naive(fmm_plan_t plan, int dtype, int m, int n, int k,
 int fmm_gemm(int dtype, int m, int n, int k, const void *A, int lda,
              const void *B, int ldb, void *C, int ldc, void *stream);
 
-/* Error flags a launch left in ws (synchronises the device first). */
+/* Error flags (FMM_ESPIN when nonzero) of the last call that used ws:
+ * every workspace (fmm_workspace_bytes / _pack, fmm_refine_workspace_bytes)
+ * starts with a status header holding them.  With ws == NULL: the flags of
+ * all calls since the last such query that allocated their workspace
+ * themselves (ws == NULL), on the current device; they are cleared by the
+ * query.  Synchronises the device first.  A flag is set when a
+ * product-order wait exceeds its time bound (5 s, or FMM_SPIN_LIMIT_MS
+ * milliseconds from the environment); C is then not reliable.  The
+ * host-pointer entries return FMM_ESPIN themselves. */
 int fmm_ws_status(const void *ws, int *flags);
 
 /* Kernel timing for benchmarks (no counterpart in the reference): while
@@ -169,6 +177,11 @@ int fmm_batch_reserve(fmm_batch_t batch, int jobs, int ws_ints);
 int fmm_batch_generation(fmm_batch_t batch);
 int fmm_batch_info(fmm_batch_t batch, int *jobs, int *units);
 int fmm_batch_status(fmm_batch_t batch, int *flags);
+/* The flags of the batch's last run that has completed, without
+ * synchronising: each run ends by copying them into host-mapped memory, so a
+ * caller that has seen the run finish (an event after fmm_batch_run) reads
+ * them here.  FMM_ESPIN when nonzero. */
+int fmm_batch_last_status(fmm_batch_t batch, int *flags);
 int fmm_batch_destroy(fmm_batch_t batch);
 
 /* ---- FP64 product through low-precision products and one correction -------

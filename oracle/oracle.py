@@ -258,3 +258,81 @@ def refine_lowp(alg_main, main: str, alg_corr, corr: str, A, B, C) -> np.ndarray
     Et = (fmm_lowp(alg_corr, AA, BB, z, corr) if alg_corr is not None
           else (z.astype(np.float64) + _lowp_product(AA, BB, corr)).astype(np.float32))
     return C + (Ct.astype(np.float64) + Et.astype(np.float64))
+
+
+def _lowp_dots(S: np.ndarray, T: np.ndarray, dtype: str) -> np.ndarray:
+    """Row-wise products P[s] = S[s, :] . T[:, s] with the rounding points of
+    _lowp_product (S: nsamp x depth, T: depth x nsamp, float32)."""
+    f = np.float64
+    if dtype == "bf16x3":
+        sh = round_bf16(S)
+        sl = round_bf16(S - sh)
+        th = round_bf16(T)
+        tl = round_bf16(T - th)
+        return (np.einsum("sk,ks->s", sh.astype(f), th.astype(f)) + np.einsum("sk,ks->s", sh.astype(f), tl.astype(f))
+                + np.einsum("sk,ks->s", sl.astype(f), th.astype(f)))
+    return np.einsum("sk,ks->s", round_op(S, dtype).astype(f), round_op(T, dtype).astype(f))
+
+
+def fmm_lowp_at(alg, A: np.ndarray, B: np.ndarray, C: np.ndarray, dtype: str, ii, jj) -> np.ndarray:
+    """fmm_lowp evaluated only at the entries (ii[s], jj[s]) of the result:
+    the same operand sums (table order, FP32, rounded to the operand type),
+    the same product order and the same C updates fl32(C + w * fl32(P)),
+    for one row of S_r and one column of T_r per sampled entry.  Lets a
+    full-size result be checked entry by entry against the restatement."""
+    A = np.asarray(A, dtype=np.float32)
+    B = np.asarray(B, dtype=np.float32)
+    ii = np.asarray(ii, dtype=np.int64)
+    jj = np.asarray(jj, dtype=np.int64)
+    f32 = np.float32
+    out = np.asarray(C, dtype=np.float32)[ii, jj].copy()
+    m, k = A.shape
+    n = B.shape[1]
+    Mx, Kx, Nx = alg.dims
+    md, kd, nd = m - m % (Mx * 8), k - k % Kx, n - n % (Nx * 4)
+    ms, ks, ns = md // Mx, kd // Kx, nd // Nx
+
+    def upd(sel, w, P):
+        out[sel] = (out[sel] + f32(w) * P[sel].astype(f32)).astype(f32)
+
+    core = (ii < md) & (jj < nd)
+    plain = ~core if (ms > 0 and ns > 0 and ks > 0) else np.ones_like(core)
+    if plain.any():   # fringe GEMMs, or a core without FMM depth: one-term products over all of k
+        P = _lowp_dots(A[ii], B[:, jj], dtype)
+        upd(plain, 1.0, P)
+    if not (ms > 0 and ns > 0 and ks > 0) or not core.any():
+        return out
+    U, V, W = alg.as_arrays()
+    I, ip = ii // ms, ii % ms
+    J, jp = jj // ns, jj % ns
+    wrow = {alg.block_of_row(2, row): row for row in range(W.shape[0])}
+    rows_w = np.array([wrow.get((int(a), int(b)), 0) for a, b in zip(I, J)])
+    kr_ = np.arange(ks)
+    for r in range(alg.rank):
+        w_s = np.where(core, W[rows_w, r], 0.0)
+        if not np.any(w_s):
+            continue
+        S = None
+        for row in range(U.shape[0]):
+            u = U[row, r]
+            if u == 0:
+                continue
+            bi, bj = alg.block_of_row(0, row)
+            x = A[(bi * ms + ip)[:, None], (bj * ks + kr_)[None, :]]
+            S = (f32(u) * x).astype(f32) if S is None else (S + f32(u) * x).astype(f32)
+        T = None
+        for row in range(V.shape[0]):
+            v = V[row, r]
+            if v == 0:
+                continue
+            bi, bj = alg.block_of_row(1, row)
+            x = B[(bi * ks + kr_)[:, None], (bj * ns + jp)[None, :]]
+            T = (f32(v) * x).astype(f32) if T is None else (T + f32(v) * x).astype(f32)
+        P = _lowp_dots(S, T, dtype)
+        for w in np.unique(w_s[w_s != 0]):
+            upd(w_s == w, w, P)
+    kr = k - kd
+    if kr > 0:
+        P = _lowp_dots(A[ii, kd:], B[kd:, jj], dtype)
+        upd(core, 1.0, P)
+    return out

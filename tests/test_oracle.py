@@ -232,3 +232,93 @@ def test_refine_restatement_integer_and_float(fmm, oracle):
     A1 = oracle.round_op(A.astype(np.float32), "bf16").astype(np.float64)
     half = C + A1 @ oracle.round_op(B.astype(np.float32), "bf16").astype(np.float64) + (A - A1) @ B
     assert np.linalg.norm(half - ref) / np.linalg.norm(exact) > 10 * e_ref
+
+
+def test_dgemm_ref_integer_kat(oracle):
+    """oracle_dgemm_ref (the j, i, p triple loop of common/dgemm/bl_dgemm_ref.c:40-47)
+    on small-integer inputs equals numpy's exact integer product, with
+    leading dimensions larger than the extents."""
+    rng = np.random.default_rng(12)
+    for (m, n, k) in [(1, 1, 1), (7, 5, 9), (33, 17, 65), (64, 3, 200)]:
+        A = aligned_f(m + 3, k)
+        B = aligned_f(k + 1, n)
+        C = aligned_f(m + 5, n)
+        A[:m, :] = rng.integers(-4, 5, (m, k))
+        B[:k, :] = rng.integers(-4, 5, (k, n))
+        C[:m, :] = rng.integers(-4, 5, (m, n))
+        C[m:, :] = 7.0
+        want = C.copy(order="F")
+        want[:m, :] += (A[:m, :].astype(np.int64) @ B[:k, :].astype(np.int64)).astype(np.float64)
+        oracle.lib().oracle_dgemm_ref(m, n, k, oracle._p(A), m + 3, oracle._p(B), k + 1, oracle._p(C), m + 5)
+        assert np.array_equal(C, want), (m, n, k)      # padding rows untouched too
+    # empty problems are no-ops (the reference's early return)
+    C0 = C.copy(order="F")
+    oracle.lib().oracle_dgemm_ref(0, n, k, oracle._p(A), 1, oracle._p(B), k + 1, oracle._p(C), m + 5)
+    assert np.array_equal(C, C0)
+
+
+def test_dgemm_ref_matches_reference_bitwise(oracle, rt):
+    """oracle_dgemm_ref and the product library's CPU bl_dgemm_ref both equal
+    the reference's own bl_dgemm_ref (built from its sources into oracle/_ref)
+    bit for bit on random float inputs: same loop order, one rounding per
+    multiply and per add."""
+    R = oracle.ref_lib()
+    if R is None:
+        pytest.skip("reference runtime not built (needs the reference sources)")
+    rng = np.random.default_rng(13)
+    for (m, n, k) in [(37, 29, 300), (8, 4, 256), (100, 100, 1), (5, 61, 513)]:
+        A = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
+        B = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+        C0 = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+        outs = []
+        for f in (R.bl_dgemm_ref, oracle.lib().oracle_dgemm_ref, rt.lib().bl_dgemm_ref):
+            C = C0.copy(order="F")
+            f(m, n, k, oracle._p(A), m, oracle._p(B), k, oracle._p(C), m)
+            outs.append(C)
+        assert np.array_equal(outs[0], outs[1]), ("oracle", m, n, k)
+        assert np.array_equal(outs[0], outs[2]), ("product bl_dgemm_ref", m, n, k)
+        assert not np.array_equal(outs[0], C0)
+
+
+def test_dgemm_ref_catches_planted_order_change(oracle):
+    """A dot product summed in another order (numpy's pairwise/BLAS sum) is not
+    bitwise equal on random floats, so the bitwise check above has teeth."""
+    rng = np.random.default_rng(14)
+    m, n, k = 37, 29, 300
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+    C = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    got = C.copy(order="F")
+    oracle.lib().oracle_dgemm_ref(m, n, k, oracle._p(A), m, oracle._p(B), k, oracle._p(got), m)
+    assert not np.array_equal(got, C + A @ B)
+    assert np.abs(got - (C + A @ B)).max() < 1e-12
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "f16", "bf16x3"])
+@pytest.mark.parametrize("spec", [(["222"], [1]), (["222"], [2]), (["323"], [1])],
+                         ids=["222L1", "222L2", "323L1"])
+def test_lowp_sampled_restatement_matches_full(fmm, oracle, spec, dtype):
+    """fmm_lowp_at equals fmm_lowp at every sampled entry (core, every fringe
+    region, k-fringe), up to one FP32 ulp where the FP64 dot products of the
+    two forms round differently; a dropped or misplaced term would differ by
+    O(1)."""
+    alg = fmm.build_algorithm(*spec)
+    Mx, Kx, Nx = alg.dims
+    rng = np.random.default_rng(15)
+    m, n, k = Mx * 8 * 3 + 5, Nx * 4 * 4 + 3, Kx * 11 + 1
+    A = oracle.round_op(rng.uniform(-1, 1, (m, k)).astype(np.float32), dtype)
+    B = oracle.round_op(rng.uniform(-1, 1, (k, n)).astype(np.float32), dtype)
+    C = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    full = oracle.fmm_lowp(alg, A, B, C, dtype)
+    ii, jj = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    got = oracle.fmm_lowp_at(alg, A, B, C, dtype, ii.ravel(), jj.ravel()).reshape(m, n)
+    ulp = np.spacing(np.abs(full).astype(np.float32))
+    assert np.all(np.abs(got - full) <= ulp), np.abs(got - full).max()
+    # planted mistake: one U coefficient of product 0 with its sign flipped
+    # changes the sampled result in the C blocks that product feeds
+    import copy
+    bad = copy.deepcopy(alg)
+    row = next(i for i, r in enumerate(bad.U) if r[0] != 0)
+    bad.U[row][0] = -bad.U[row][0]
+    got_bad = oracle.fmm_lowp_at(bad, A, B, C, dtype, ii.ravel(), jj.ravel()).reshape(m, n)
+    assert np.abs(got_bad - full).max() > 1e-2

@@ -76,3 +76,28 @@ def test_continuous_batching_overlaps_lanes():
     assert len(srv.done) == len(reqs)
     assert max(srv.batch_sizes) > 1          # requests share launches
     assert ex.max_running >= 2               # launches overlap on the device
+
+
+def test_launch_error_flags_reach_requests():
+    """The executor's per-launch error flags (a batch's product-order wait
+    that timed out) mark exactly the requests of that launch as failed."""
+    class Faulty(S.SimExecutor):
+        def __init__(self, *a, **k):
+            super().__init__(*a, **k)
+            self.n = 0
+
+        def launch(self, ln, reqs):
+            super().launch(ln, reqs)
+            self.n += 1
+            ln.state["error"] = 1 if self.n == 2 else 0     # the second launch reports a flag
+
+    reqs = mix(30, seed=5)
+    ex = Faulty(clock=__import__("time").perf_counter, flops_per_s=1e9, launch_s=1e-4)
+    srv = S.Server(ex, hbm_budget=1e12, lanes_per_dtype=1, max_batch=4)
+    launched = []
+    orig = ex.launch
+    ex.launch = lambda ln, rs: (launched.append(list(rs)), orig(ln, rs))
+    srv.serve(reqs, timeout=60)
+    bad = {r.rid for r in launched[1]}
+    assert {r.rid for r in srv.done if r.error} == bad
+    assert srv.stats()["failed"] == len(bad) > 0
