@@ -238,7 +238,7 @@ def test_batch_1024_problems(torch, fmm, rt):
     for i in sample:
         A, B, C = ops[i]
         a, t = choices[i]
-        single = C0[i].t().contiguous().t()
+        single = rt.as_colmajor(C0[i])
         plan = tuner.plan(a)
         if plan is None:
             rt.gemm(A, B, single)
