@@ -8,6 +8,7 @@
 #include "fmm_batch.hpp"
 #include "fmm_refine.hpp"
 #include "fmm_host.hpp"
+#include "fmm_noalloc.hpp"
 #include "fmm_amd.h"
 
 struct fmm_plan_s {
@@ -19,12 +20,6 @@ struct fmm_batch_s {
 };
 
 namespace fmm {
-
-inline bool bad_dims(int m, int n, int k, int lda, int ldb, int ldc) {
-  if (m < 0 || n < 0 || k < 0) return true;
-  if (lda < std::max(1, m) || ldb < std::max(1, k) || ldc < std::max(1, m)) return true;
-  return false;
-}
 
 // err_dst: where the flags of this call's ordered launch are OR-ed (a word
 // of the caller's status header; nullptr: they stay in ws, or go to the
@@ -257,6 +252,12 @@ int fmm_naive(fmm_plan_t plan, int dtype, int m, int n, int k, const void* A, in
   return fmm_run(plan, FMM_PACK_NAIVE, dtype, m, n, k, A, lda, B, ldb, C, ldc, ws, ws_bytes, stream);
 }
 
+int fmm_abc_noalloc(fmm_plan_t plan, int dtype, int m, int n, int k, const void* A, int lda, const void* B, int ldb,
+                    void* C, int ldc, void* stream) {
+  if (!plan) return fmm::kEInval;
+  return fmm::run_abc_noalloc(&plan->p, dtype, m, n, k, A, lda, B, ldb, C, ldc, static_cast<hipStream_t>(stream));
+}
+
 int fmm_gemm(int dtype, int m, int n, int k, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
              void* stream) {
   return fmm::run_gemm(dtype, m, n, k, A, lda, B, ldb, C, ldc, static_cast<hipStream_t>(stream));
@@ -265,8 +266,11 @@ int fmm_gemm(int dtype, int m, int n, int k, const void* A, int lda, const void*
 int fmm_ws_status(const void* ws, int* flags) {
   if (!flags) return fmm::kEInval;
   if (!ws) {
-    const int rc = fmm::take_sticky(flags);
+    int rc = fmm::take_sticky(flags);
+    int na = 0;
+    if (rc == fmm::kOK) rc = fmm::noalloc_take_flags(&na);
     if (rc != fmm::kOK) return rc;
+    *flags |= na;
     return *flags ? fmm::kESpin : fmm::kOK;
   }
   if (hipDeviceSynchronize() != hipSuccess) return fmm::kEHip;

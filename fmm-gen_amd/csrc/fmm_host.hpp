@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -27,7 +28,14 @@ enum Status : int {
   kENoMem = -3,
   kEUnsupported = -4,
   kESpin = -5,
+  kETooBig = -6,
 };
+
+inline bool bad_dims(int m, int n, int k, int lda, int ldb, int ldc) {
+  if (m < 0 || n < 0 || k < 0) return true;
+  if (lda < std::max(1, m) || ldb < std::max(1, k) || ldc < std::max(1, m)) return true;
+  return false;
+}
 
 // XCD-local C hand-over between ordered products (see release_c); 0 keeps
 // the agent-scope release everywhere.
@@ -80,6 +88,7 @@ struct Table {
 };
 
 struct Plan {
+  unsigned long long uid = 0;   // unique per built plan in this process (never 0)
   int nlev = 0;
   std::vector<int> dims;   // 3*nlev
   int Mx = 1, Kx = 1, Nx = 1, R = 0;
@@ -129,6 +138,8 @@ inline void decode_row(const std::vector<int>& dims, int which, int row, int* I,
 inline int build_plan(Plan* P, int nlev, const int* dims, int R, const double* U, const double* V,
                       const double* W) {
   if (!P || nlev <= 0 || !dims || R <= 0 || !U || !V || !W) return kEInval;
+  static std::atomic<unsigned long long> next_uid{1};
+  P->uid = next_uid.fetch_add(1);
   P->nlev = nlev;
   P->dims.assign(dims, dims + 3 * nlev);
   P->Mx = P->Kx = P->Nx = 1;
@@ -368,11 +379,13 @@ inline long long spin_ticks(const DevInfo& d) {
 }
 
 // One-time probe: launch 256 workgroups and check that workgroup i reports
-// XCC id i % 8 (the dispatch pattern XCD-local hand-over relies on).
-inline bool probe_xcd_round_robin() {
-  const int n = 256;
-  int* d = nullptr;
-  if (hipMalloc(&d, n * sizeof(int)) != hipSuccess) return false;
+// XCC id i % 8 (the dispatch pattern XCD-local hand-over relies on).  `buf`:
+// kXcdProbeInts device ints to use, or nullptr (allocated and freed here).
+constexpr int kXcdProbeInts = 256;
+inline bool probe_xcd_round_robin(int* buf) {
+  const int n = kXcdProbeInts;
+  int* d = buf;
+  if (!d && hipMalloc(&d, n * sizeof(int)) != hipSuccess) return false;
   bool ok = false;
   xcc_probe_kernel<<<dim3(n), dim3(64), 0, 0>>>(d);
   std::vector<int> h(n, -1);
@@ -381,8 +394,28 @@ inline bool probe_xcd_round_robin() {
     ok = true;
     for (int i = 0; i < n; ++i) ok = ok && h[i] == i % kMaxQueues;
   }
-  (void)hipFree(d);
+  if (!buf) (void)hipFree(d);
   return ok;
+}
+
+// Per-device result of the probe: -1 not yet probed, else 0 / 1.
+struct XcdProbe {
+  std::mutex mu;
+  std::vector<int> rr;
+};
+inline XcdProbe& xcd_probe() {
+  static XcdProbe p;
+  return p;
+}
+
+// Probe the current device once (with `buf` as in probe_xcd_round_robin);
+// returns whether workgroups are placed round robin over the XCDs.
+inline bool ensure_xcd_probe(int dev, int* buf) {
+  XcdProbe& p = xcd_probe();
+  std::lock_guard<std::mutex> g(p.mu);
+  if ((int)p.rr.size() <= dev) p.rr.resize(dev + 1, -1);
+  if (p.rr[dev] < 0) p.rr[dev] = probe_xcd_round_robin(buf) ? 1 : 0;
+  return p.rr[dev] == 1;
 }
 
 // Launch properties of `kernel` (nt threads per workgroup) on the current
@@ -395,7 +428,6 @@ inline int dev_info(const void* kernel, int nt, DevInfo* out) {
   };
   static std::mutex mu;
   static std::vector<Entry> cache;
-  static std::vector<int> rr;   // per device: -1 unknown, 0/1 probed
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return kEHip;
   std::lock_guard<std::mutex> g(mu);
@@ -408,14 +440,13 @@ inline int dev_info(const void* kernel, int nt, DevInfo* out) {
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return kEHip;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, nt, 0) != hipSuccess) return kEHip;
-  if ((int)rr.size() <= dev) rr.resize(dev + 1, -1);
-  if (rr[dev] < 0) rr[dev] = probe_xcd_round_robin() ? 1 : 0;
+  const bool rr = ensure_xcd_probe(dev, nullptr);
   Entry e;
   e.dev = dev;
   e.kernel = kernel;
   e.d.cus = cus;
   e.d.blocks_per_cu = std::max(nb, 1);
-  e.d.xcd_round_robin = rr[dev] == 1;
+  e.d.xcd_round_robin = rr;
   e.d.ticks_per_ms = (long long)khz;
   e.d.spin_ticks = (long long)khz * 1000LL * 5LL;   // 5 s
   cache.push_back(e);

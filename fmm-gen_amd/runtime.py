@@ -26,7 +26,7 @@ FMM_F64, FMM_F32, FMM_BF16, FMM_F16, FMM_BF16X3 = 0, 1, 2, 3, 4
 DTYPE_NAMES = {FMM_F64: "f64", FMM_F32: "f32", FMM_BF16: "bf16", FMM_F16: "f16", FMM_BF16X3: "bf16x3"}
 PACKS = {"abc": 0, "ab": 1, "naive": 2}
 STATUS = {0: "FMM_OK", -1: "FMM_EINVAL", -2: "FMM_EHIP", -3: "FMM_ENOMEM",
-          -4: "FMM_EUNSUPPORTED", -5: "FMM_ESPIN"}
+          -4: "FMM_EUNSUPPORTED", -5: "FMM_ESPIN", -6: "FMM_ETOOBIG"}
 
 EXPORTS = ("fmm_plan_create", "fmm_plan_destroy", "fmm_plan_info", "fmm_workspace_bytes",
            "fmm_workspace_bytes_pack", "fmm_run", "fmm_abc", "fmm_ab", "fmm_naive", "fmm_gemm",
@@ -34,7 +34,7 @@ EXPORTS = ("fmm_plan_create", "fmm_plan_destroy", "fmm_plan_info", "fmm_workspac
            "bl_finalize", "fmm_batch_create", "fmm_batch_set", "fmm_batch_run", "fmm_batch_variant",
            "fmm_batch_info", "fmm_batch_status", "fmm_batch_destroy", "fmm_batch_reserve",
            "fmm_batch_generation", "fmm_batch_last_status", "fmm_refine", "fmm_refine_workspace_bytes",
-           "fmm_timing", "fmm_timing_last_ms")
+           "fmm_timing", "fmm_timing_last_ms", "fmm_abc_noalloc")
 
 
 class FmmError(RuntimeError):
@@ -59,6 +59,8 @@ def _declare(L, has_job: bool = False):
     for name in ("fmm_abc", "fmm_ab", "fmm_naive"):
         getattr(L, name).argtypes = [vp, i, i, i, i, vp, i, vp, i, vp, i, vp, ct.c_size_t, vp]
         getattr(L, name).restype = i
+    L.fmm_abc_noalloc.argtypes = [vp, i, i, i, i, vp, i, vp, i, vp, i, vp]
+    L.fmm_abc_noalloc.restype = i
     L.fmm_run.argtypes = [vp, i, i, i, i, i, vp, i, vp, i, vp, i, vp, ct.c_size_t, vp]
     L.fmm_run.restype = i
     L.fmm_host_run.argtypes = [vp, i, i, i, i, dp, i, dp, i, dp, i]
@@ -224,6 +226,16 @@ class Plan:
         check(self._lib.fmm_abc(self.handle, dt, m, n, k, A.data_ptr(), _ld(A), B.data_ptr(),
                                 _ld(B), C.data_ptr(), _ld(C), wptr, wbytes, _stream_ptr(stream)),
               f"fmm_abc({DTYPE_NAMES[dt]})")
+
+    def abc_noalloc(self, A, B, C, stream=None, split: bool = False) -> None:
+        """C += A @ B by the ABC FMM with no device allocation (the library's
+        static scratch array holds the workspace; calls overlapping on one
+        GPU race)."""
+        m, n, k = _check_operands(A, B, C)
+        dt = dtype_code(A, B, C, split)
+        check(self._lib.fmm_abc_noalloc(self.handle, dt, m, n, k, A.data_ptr(), _ld(A), B.data_ptr(),
+                                        _ld(B), C.data_ptr(), _ld(C), _stream_ptr(stream)),
+              f"fmm_abc_noalloc({DTYPE_NAMES[dt]})")
 
     def host_run(self, A: np.ndarray, B: np.ndarray, C: np.ndarray, pack: str = "abc") -> None:
         """Host-pointer drop-in on Fortran-ordered float64 numpy arrays."""

@@ -28,6 +28,7 @@ extern "C" {
 #define FMM_ENOMEM       (-3)  /* device or host allocation failed               */
 #define FMM_EUNSUPPORTED (-4)  /* dtype / variant not built                      */
 #define FMM_ESPIN        (-5)  /* a product-order wait exceeded its time bound   */
+#define FMM_ETOOBIG      (-6)  /* larger than fmm_abc_noalloc's fixed scratch    */
 
 /* Element types (dtype argument).  A and B hold the named type; C is FP64
  * for FMM_F64 and FP32 for every other type.  Operand sums are formed in the
@@ -111,6 +112,25 @@ int fmm_naive(fmm_plan_t plan, int dtype, int m, int n, int k,
               const void *A, int lda, const void *B, int ldb, void *C, int ldc,
               void *ws, size_t ws_bytes, void *stream);
 
+/* fmm_abc without any device allocation.  No counterpart in the reference;
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments as fmm_abc minus the workspace, same kernels, results equal to
+ * fmm_abc's bit for bit.  The workspace (product-order counters, one per C
+ * tile of the leaf block) comes from a static device array of the library
+ * (1.25 MiB per GPU, held until the process ends): at most 65,272 C tiles of
+ * 128 x 128 per leaf block (FP64; leaf blocks up to ~32,000 x 32,000), and
+ * -- when the plan has not run on this device through another entry yet --
+ * at most 1 MiB of its product tables.  Larger problems return FMM_ETOOBIG.
+ * The first call on a device sets the array up (symbol lookup, a fill, a
+ * one-off placement probe; it synchronises the device).  All calls on one
+ * GPU share the array: calls that overlap on one GPU (other streams or host
+ * threads) race and their results are undefined.  Order-wait flags are read
+ * by fmm_ws_status(NULL, ...). */
+int fmm_abc_noalloc(fmm_plan_t plan, int dtype, int m, int n, int k,
+                    const void *A, int lda, const void *B, int ldb, void *C,
+                    int ldc, void *stream);
+
 /* Plain GEMM C += A*B on the same kernel (identity plan), any dtype above.
  * Replaces: bl_dgemm (common/dgemm/my_dgemm.c:115-210). */
 int fmm_gemm(int dtype, int m, int n, int k, const void *A, int lda,
@@ -121,7 +141,8 @@ int fmm_gemm(int dtype, int m, int n, int k, const void *A, int lda,
  * starts with a status header holding them.  With ws == NULL: the flags of
  * all calls since the last such query that allocated their workspace
  * themselves (ws == NULL), on the current device; they are cleared by the
- * query.  Synchronises the device first.  A flag is set when a
+ * query, together with those of fmm_abc_noalloc calls.  Synchronises the
+ * device first.  A flag is set when a
  * product-order wait exceeds its time bound (5 s, or FMM_SPIN_LIMIT_MS
  * milliseconds from the environment); C is then not reliable.  The
  * host-pointer entries return FMM_ESPIN themselves. */

@@ -96,3 +96,16 @@ def test_batch_argument_validation_before_device(fmm, rt):
     assert lib.fmm_batch_create(ct.byref(h), 9, 1, probs, 0, plans) == -4      # unknown dtype
     assert lib.fmm_batch_create(ct.byref(h), 1, -1, probs, 0, plans) == -1
     assert lib.fmm_batch_run(None, None) == -1
+
+
+def test_noalloc_rejects_oversize_on_host(fmm, rt):
+    """fmm_abc_noalloc checks its fixed scratch limits on the host, before any
+    device call: a leaf block of more than 65,272 C tiles returns FMM_ETOOBIG."""
+    lib = rt.lib()
+    p = rt.Plan(fmm.build_algorithm(["222"], [1]))
+    big = 2 * 33000        # leaf 33000 x 33000: 258^2 = 66,564 tiles of 128 x 128
+    assert lib.fmm_abc_noalloc(p.handle, 0, big, big, 2, None, big, None, 2, None, big, None) == -6
+    assert lib.fmm_abc_noalloc(p.handle, 0, -1, 4, 4, None, 4, None, 4, None, 4, None) == -1
+    assert lib.fmm_abc_noalloc(p.handle, 9, 8, 4, 4, None, 8, None, 4, None, 8, None) == -4
+    assert lib.fmm_abc_noalloc(p.handle, 0, 0, 4, 4, None, 1, None, 4, None, 1, None) == 0
+    p.close()
