@@ -68,7 +68,9 @@ inline int run_fmm(Plan* P, int pack, int dtype, int m, int n, int k, const void
   if (!dtype_geo(dtype, &G)) return kEUnsupported;
   if (m == 0 || n == 0 || k == 0) return kOK;
   PackLayout Lo;
-  if (pack == kPackABC || !pack_layout(*P, pack, m, n, k, &Lo, G.sin, G.sc, naive_fused(dtype)))
+  const int ks = (k - k % P->Kx) / P->Kx;
+  if (pack == kPackABC ||
+      !pack_layout(*P, pack, m, n, k, &Lo, G.sin, G.sc, naive_fused(dtype), use_cs(*P, dtype, ks)))
     return run_abc(P, dtype, m, n, k, A, lda, B, ldb, C, ldc, ws, ws_bytes, s);
   void* w = ws;
   bool own = false;
@@ -89,7 +91,8 @@ inline int run_fmm(Plan* P, int pack, int dtype, int m, int n, int k, const void
 inline size_t workspace_bytes(Plan* P, int pack, int m, int n, int k) {
   if (m <= 0 || n <= 0 || k <= 0) return 0;
   PackLayout Lo;
-  if (pack != kPackABC && pack_layout(*P, pack, m, n, k, &Lo)) return Lo.bytes;
+  const int ks = (k - k % P->Kx) / P->Kx;
+  if (pack != kPackABC && pack_layout(*P, pack, m, n, k, &Lo, 8, 8, false, use_cs(*P, kF64, ks))) return Lo.bytes;
   JobList JL;
   add_fmm_jobs(JL, *P, nullptr, nullptr, nullptr, nullptr, m, n, k, nullptr, std::max(m, 1), nullptr,
                std::max(k, 1), nullptr, std::max(m, 1));

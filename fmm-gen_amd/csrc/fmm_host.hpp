@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "fmm_core.hpp"
+#include "fmm_cs.hpp"
 #include "fmm_lowp.hpp"
 
 namespace fmm {
@@ -107,6 +108,11 @@ struct Plan {
   Table naive;
   // AB: product r reads its A/B terms as in ABC and writes M_r.
   Table ab;
+  // Naive, C-stationary launch: product r reads S_r / T_r (term bi = -1,
+  // bj = r) or, for an operand with a single term, that block of A / B
+  // directly, and carries the W terms of `main`; then the k-fringe products
+  // of `main`.
+  Table naive_cs;
   // For every C block (I*Nx + J, grid order) the products feeding it, in
   // product order, with their W coefficients (the M_Add lists).
   DevVec<int> cptr, cprod;
@@ -216,6 +222,24 @@ inline int build_plan(Plan* P, int nlev, const int* dims, int R, const double* U
   P->naive.terms.h.clear();
   P->ab.prods.h.clear();
   P->ab.terms.h.clear();
+  P->naive_cs.prods.h.clear();
+  P->naive_cs.terms.h.clear();
+  for (int r = 0; r < R + P->Mx * P->Nx; ++r) {
+    const Prod& o = prods[r];
+    std::vector<Term>& ct = P->naive_cs.terms.h;
+    Prod c{};
+    c.a0 = (int)ct.size();
+    ct.push_back(r >= R || o.na == 1 ? terms[o.a0] : Term{1.0, -1, r});
+    c.na = 1;
+    c.b0 = (int)ct.size();
+    ct.push_back(r >= R || o.nb == 1 ? terms[o.b0] : Term{1.0, -1, r});
+    c.nb = 1;
+    c.c0 = (int)ct.size();
+    for (int t = 0; t < o.nc; ++t) ct.push_back(terms[o.c0 + t]);
+    c.nc = o.nc;
+    c.fringe = o.fringe;
+    P->naive_cs.prods.h.push_back(c);
+  }
   for (int r = 0; r < R; ++r) {
     std::vector<Term>& dt = P->diag.terms.h;
     Prod d{};
@@ -336,7 +360,7 @@ inline int device_clists(Plan* P, int** ptr, int** prod, double** w) {
 
 inline void release_plan(Plan* P) {
   for (Table* T : {&P->main, &P->diag, &P->diag_by[0], &P->diag_by[1], &P->diag_by[2], &P->diag_by[3], &P->naive,
-                   &P->ab}) {
+                   &P->ab, &P->naive_cs}) {
     T->prods.release();
     T->terms.release();
   }
@@ -826,6 +850,85 @@ inline int launch_jobs_dtype(int dtype, const JobList& JL, int max_terms, int* w
   }
 }
 
+// --- C-stationary launches (FP64, one C tile of every C block on chip) -------
+//
+// The FP64 Naive pack type runs its products C-stationary (fmm_cs.hpp) when
+// the algorithm has at most 9 C blocks and the leaf is shallow: measured at
+// m = n = 14400 (k = 480, leaf depth 160-240) it beats the ordered product
+// launch + M_Add by 12-35% (<2,2,2>, <2,3,2>, <3,3,3>), from k = 960 (leaf
+// depth 320-480) the ordered launch is faster.  With operand sums (ABC) the
+// C-stationary kernel measured slower than the ordered one at every depth,
+// so ABC keeps the ordered kernel.  FMM_CS=0 / 1 in the environment forces
+// the ordered / the C-stationary product launch for Naive (when the C
+// blocks fit).
+constexpr int kCsMaxBlocks = 9;
+inline bool cs_fits(const Plan& P) { return P.Mx * P.Nx <= kCsMaxBlocks; }
+#ifndef FMM_CS_MAX_DEPTH
+#define FMM_CS_MAX_DEPTH 256
+#endif
+inline bool use_cs(const Plan& P, int dtype, int ks) {
+  if (dtype != kF64 || !cs_fits(P) || ks <= 0) return false;
+  const char* e = getenv("FMM_CS");
+  if (e && *e) return atoi(e) != 0;
+  return ks <= FMM_CS_MAX_DEPTH;
+}
+
+template <class Cfg>
+inline int launch_cs_k(CsJob J, hipStream_t stream) {
+  DevInfo di;
+  int rc = dev_info(reinterpret_cast<const void*>(&cs_f64_kernel<Cfg>), Cfg::NT, &di);
+  if (rc != kOK) return rc;
+  J.tiles_m = tiles(J.ms, Cfg::TM);
+  J.tiles_n = tiles(J.ns, Cfg::TN);
+  const int units = J.tiles_m * J.tiles_n;
+  if (units <= 0 || J.nprod <= 0) return kOK;
+  int grid = std::max(1, std::min(units, di.cus * di.blocks_per_cu));
+  int nq = 1;
+  if (grid >= 8 * kMaxQueues) {
+    grid -= grid % kMaxQueues;
+    nq = kMaxQueues;
+  }
+  LaunchTiming& tm = launch_timing();
+  if (tm.on && tm.ev[0]) (void)hipEventRecord(tm.ev[0], stream);
+  hipLaunchKernelGGL(cs_f64_kernel<Cfg>, dim3(grid), dim3(Cfg::NT), 0, stream, J, nq);
+  if (hipGetLastError() != hipSuccess) return kEHip;
+  if (tm.on && tm.ev[1]) (void)hipEventRecord(tm.ev[1], stream);
+  return kOK;
+}
+
+// Single-term products (Naive over S / T, or single-term A / B blocks) by
+// C-block count: <= 4 blocks 128 x 64 positions, <= 9 blocks 64 x 64.
+inline int launch_cs(const CsJob& J, hipStream_t stream) {
+  const int ncb = J.mx * J.nx;
+  if (ncb > kCsMaxBlocks) return kEInval;
+  if (ncb <= 4) return launch_cs_k<CsCfg<4, 32, 1>>(J, stream);
+  return launch_cs_k<CsCfg<9, 32, 1>>(J, stream);
+}
+
+// The C-stationary core job of an FMM call (its geometry from the partition).
+inline CsJob cs_core_job(const Plan& P, const Prod* dp, const Term* dt, int m, int n, int k, const void* A,
+                         long long lda, const void* B, long long ldb, void* C, long long ldc) {
+  CsJob j{};
+  const int md = m - m % (P.Mx * kMR), kd = k - k % P.Kx, nd = n - n % (P.Nx * kNR);
+  j.A = static_cast<const double*>(A);
+  j.B = static_cast<const double*>(B);
+  j.C = static_cast<double*>(C);
+  j.lda = lda;
+  j.ldb = ldb;
+  j.ldc = ldc;
+  j.ms = md / P.Mx;
+  j.ns = nd / P.Nx;
+  j.ks = kd / P.Kx;
+  j.kr = k - kd;
+  j.nprod = P.R + (j.kr > 0 ? P.Mx * P.Nx : 0);
+  j.mx = P.Mx;
+  j.nx = P.Nx;
+  j.vec = vec_flags(kGeoF64, A, lda, B, ldb, j.ks, C, ldc);
+  j.prods = dp;
+  j.terms = dt;
+  return j;
+}
+
 // --- AB and Naive pack types ---------------------------------------------
 //
 // Pack types of the reference (meta/ab_gen_mix.py, meta/naive_gen_mix.py):
@@ -863,8 +966,9 @@ inline size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
 // of a unit, M_Add streams).
 inline bool naive_fused(int dtype) { return dtype != kF64; }
 
+// cs: FP64 Naive with a C-stationary product launch (use_cs): S and T only.
 inline bool pack_layout(const Plan& P, int pack, int m, int n, int k, PackLayout* L, int sin = 8, int sc = 8,
-                        bool fused = false) {
+                        bool fused = false, bool cs = false) {
   PackLayout o;
   o.md = m - m % (P.Mx * kMR);
   o.kd = k - k % P.Kx;
@@ -892,7 +996,7 @@ inline bool pack_layout(const Plan& P, int pack, int m, int n, int k, PackLayout
     o.t_off = off;
     off += round256((size_t)sin * (size_t)o.ldt * o.ns * P.R);
   }
-  if (!(pack == kPackNaive && fused)) {
+  if (!(pack == kPackNaive && (fused || cs))) {
     o.m_off = off;
     off += round256((size_t)sc * (size_t)o.ldm * o.ns * o.nlive);
   }
@@ -1049,6 +1153,47 @@ inline int launch_naive_fused(Plan* P, int dtype, const PackLayout& Lo, int m, i
   return launch_jobs_dtype(dtype, JF, 1, nullptr, stream, true);
 }
 
+// One FP64 Naive call with a C-stationary product launch: the pre-sums of
+// the operands with more than one term into S, T; the right and bottom
+// fringe GEMMs; then every product (over S_r / T_r, or A / B blocks for
+// single-term operands) and the k-fringe products per position, C updated in
+// registers in product order and written once -- the C updates of ABC in the
+// same order, so the result equals ABC's bit for bit.
+inline int launch_naive_cs(Plan* P, const PackLayout& Lo, int m, int n, int k, const void* A, long long lda,
+                           const void* B, long long ldb, void* C, long long ldc, void* ws, hipStream_t stream) {
+  Prod *dp, *cp, *ip;
+  Term *dt, *ct, *it;
+  int rc = device_tables(P, P->main, &dp, &dt);
+  if (rc == kOK) rc = device_tables(P, P->naive_cs, &cp, &ct);
+  if (rc == kOK) rc = device_tables(identity_plan(), &ip, &it);
+  if (rc != kOK) return rc;
+  char* w = static_cast<char*>(ws);
+  hipLaunchKernelGGL(ws_clear_kernel, dim3(1), dim3(256), 0, stream, reinterpret_cast<int*>(w + Lo.o_off), Lo.o_ints);
+  if (hipGetLastError() != hipSuccess) return kEHip;
+  void* S = w + Lo.s_off;
+  void* T = w + Lo.t_off;
+  rc = launch_presums(P, kF64, Lo, dp, dt, A, lda, B, ldb, S, T, stream, true);
+  if (rc != kOK) return rc;
+  JobList JL;
+  add_gemm_job(JL, kGeoF64, ip, it, m, n - Lo.nd, k, A, lda, eoff(B, (long long)Lo.nd * ldb, 8), ldb,
+               eoff(C, (long long)Lo.nd * ldc, 8), ldc);
+  add_gemm_job(JL, kGeoF64, ip, it, m - Lo.md, Lo.nd, k, eoff(A, Lo.md, 8), lda, B, ldb, eoff(C, Lo.md, 8), ldc);
+  if (JL.total_units > 0) {
+    LaunchTiming& tm = launch_timing();
+    const bool on = tm.on;
+    tm.on = false;   // time the product launch below, not the fringe GEMMs
+    rc = launch_jobs_dtype(kF64, JL, 1, nullptr, stream, true);
+    tm.on = on;
+    if (rc != kOK) return rc;
+  }
+  CsJob J = cs_core_job(*P, cp, ct, m, n, k, A, lda, B, ldb, C, ldc);
+  J.S = static_cast<const double*>(S);
+  J.T = static_cast<const double*>(T);
+  J.lds = Lo.lds;
+  J.ldt = Lo.ldt;
+  return launch_cs(J, stream);
+}
+
 // One AB or (M-writing) Naive call C += A*B on `stream`: [pre-sums,] one
 // product launch (the products into M -- AB with its operand sums formed
 // while packing, Naive over S, T --, the k-fringe products into M, the right
@@ -1061,6 +1206,8 @@ inline int launch_pack(Plan* P, int pack, int dtype, const PackLayout& Lo, int m
   const bool naive = pack == kPackNaive;
   if (naive && naive_fused(dtype))
     return launch_naive_fused(P, dtype, Lo, m, n, k, A, lda, B, ldb, C, ldc, ws, stream);
+  if (naive && use_cs(*P, dtype, Lo.ks))
+    return launch_naive_cs(P, Lo, m, n, k, A, lda, B, ldb, C, ldc, ws, stream);
   KGeo G;
   if (!dtype_geo(dtype, &G)) return kEUnsupported;
   Prod *dp, *xp, *ip;

@@ -8,7 +8,15 @@ order, then the k-fringe products), so on any input their results must be
 bitwise equal to ABC's.  Integer known-answer cases are also checked
 bit-exact against numpy and against the oracle's restatements of the
 reference's ab / naive generators.
+
+FP64 Naive has two product launches, picked by leaf depth: ordered products
+into M plus M_Add, and the C-stationary launch (every product of a tile
+position applied to register-resident C tiles).  "naive" forces the first
+(FMM_CS=0), "naive-cs" the second (FMM_CS=1, algorithms with <= 9 C blocks).
 """
+import contextlib
+import os
+
 import numpy as np
 import pytest
 
@@ -20,7 +28,27 @@ pytestmark = pytest.mark.gpu
 SPECS = [(["222"], [1]), (["222"], [2]), (["222", "323"], [1, 1]), (["333"], [1]),
          (["232"], [1]), (["323"], [1])]
 IDS = ["222L1", "222L2", "222x323", "333L1", "232L1", "323L1"]
-PACKS = ["ab", "naive"]
+PACKS = ["ab", "naive", "naive-cs"]
+CS_MODE = {"naive": "0", "naive-cs": "1"}
+
+
+def base_pack(pack):
+    return "naive" if pack == "naive-cs" else pack
+
+
+@contextlib.contextmanager
+def cs_mode(pack):
+    """FMM_CS for the duration of one call of test pack type `pack`."""
+    old = os.environ.get("FMM_CS")
+    if pack in CS_MODE:
+        os.environ["FMM_CS"] = CS_MODE[pack]
+    try:
+        yield
+    finally:
+        if old is None:
+            os.environ.pop("FMM_CS", None)
+        else:
+            os.environ["FMM_CS"] = old
 
 
 @pytest.fixture(scope="module")
@@ -43,8 +71,9 @@ def run(torch, rt, plan, pack, A, B, C, ldpad=0, ws=None):
     dB = to_dev(torch, rt, B, ld=k + ldpad if ldpad else None)
     dC = to_dev(torch, rt, C, ld=m + ldpad if ldpad else None)
     print(f"pack={pack} m={m} n={B.shape[1]} k={k} ldpad={ldpad}", flush=True)
-    plan.run(dA, dB, dC, pack, ws=ws)
-    torch.cuda.synchronize()
+    with cs_mode(pack):
+        plan.run(dA, dB, dC, base_pack(pack), ws=ws)
+        torch.cuda.synchronize()
     return dC.cpu().numpy()
 
 
@@ -64,7 +93,7 @@ def test_pack_kat_bit_exact(torch, fmm, rt, oracle, spec, pack):
         got = run(torch, rt, plan, pack, A, B, C)
         assert np.array_equal(got, want), (spec, pack, m, n, k, float(np.abs(got - want).max()))
         ref = C.copy(order="F")
-        oracle.fmm(alg, A, B, ref, pack)
+        oracle.fmm(alg, A, B, ref, base_pack(pack))
         assert np.array_equal(got, ref)
 
 
@@ -104,12 +133,14 @@ def test_pack_workspace(torch, fmm, rt, pack):
     alg = fmm.build_algorithm(["333"], [1])
     plan = rt.Plan(alg)
     m, n, k = 720, 600, 301
-    nbytes = plan.workspace_bytes(m, n, k, pack)
+    with cs_mode(pack):
+        nbytes = plan.workspace_bytes(m, n, k, base_pack(pack))
     ms, ks, ns = 240, 100, 200
     Mblocks = alg.rank + 9            # products + k-fringe products
     header = -(-4 * 264 // 256) * 256   # status header (error word where an ABC workspace has it)
-    want = header + -(-8 * ms * ns * Mblocks // 256) * 256
-    if pack == "naive":
+    # FP64 Naive: M blocks with the ordered product launch, none C-stationary
+    want = header + (0 if pack == "naive-cs" else -(-8 * ms * ns * Mblocks // 256) * 256)
+    if base_pack(pack) == "naive":
         ldt = -(-ks // 8) * 8             # T's leading dimension: a multiple of 8 elements
         want += -(-8 * ms * ks * alg.rank // 256) * 256 + -(-8 * ldt * ns * alg.rank // 256) * 256
     assert nbytes == want
@@ -131,7 +162,8 @@ def test_pack_host_dropin(torch, fmm, rt, pack):
     rng = np.random.default_rng(19)
     A, B, C = int_mats(rng, 150, 90, 61)
     got = C.copy(order="F")
-    plan.host_run(A, B, got, pack)
+    with cs_mode(pack):
+        plan.host_run(A, B, got, base_pack(pack))
     assert np.array_equal(got, C + A @ B)
 
 
