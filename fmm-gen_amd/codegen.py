@@ -28,7 +28,8 @@ from typing import List, Sequence
 from . import build as B
 from .coeffs import Algorithm
 
-HEADERS = ["fmm_core.hpp", "fmm_host.hpp", "fmm_capi.hpp"]
+# every kernel / runtime header: a job directory builds on its own (makefile)
+HEADERS = sorted(f for f in os.listdir(B.CSRC) if f.endswith(".hpp"))
 
 
 def _fmt(x) -> str:

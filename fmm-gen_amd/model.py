@@ -26,8 +26,17 @@ import os
 from dataclasses import dataclass
 from typing import Dict, List, Sequence, Tuple
 
-from .alg_dic import alg_dic, alg_set
-from .coeffs import Algorithm, compose, load_algorithm
+if __package__ in (None, ""):
+    # run as a script (the reference's `python model_gen.py`): import through the package
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from __graft_entry__ import load_package  # noqa: E402
+    load_package()
+    from fmm_gen_amd.alg_dic import alg_dic, alg_set  # noqa: E402
+    from fmm_gen_amd.coeffs import Algorithm, compose, load_algorithm  # noqa: E402
+else:
+    from .alg_dic import alg_dic, alg_set
+    from .coeffs import Algorithm, compose, load_algorithm
 
 GEMM_ROW = [1, 0, 0, 0, 1, 1, 1, 1, 1, 1]
 PACKS = ("abc", "ab", "naive")
@@ -168,9 +177,18 @@ def generate_result_mat(coef: List[List[float]], test_type: str, mach: Machine =
     return out
 
 
+MACHINES = {m.name: m for m in (REFERENCE, MI355X_F64, MI355X_F32, MI355X_BF16)}
+
+
+def csv_suffix(mach: Machine) -> str:
+    """The reference names its files ..._1core.csv (one AVX core); the MI355X
+    parameterisations are named for the machine."""
+    return "1core" if mach is REFERENCE else mach.name
+
+
 def csv_files(out_dir: str, mach: Machine = REFERENCE) -> List[str]:
-    """The reference's 18 CSVs: model_<shape>_<L>level_<pack>_1core.csv, rows =
-    sizes, columns = dim, gemm, 23 algorithms (model_gen.py:177-249)."""
+    """The reference's 18 CSVs: model_<shape>_<L>level_<pack>_<suffix>.csv,
+    rows = sizes, columns = dim, gemm, 23 algorithms (model_gen.py:177-249)."""
     os.makedirs(out_dir, exist_ok=True)
     coef = generate_coefficient_mat()
     n = len(alg_set)
@@ -178,7 +196,7 @@ def csv_files(out_dir: str, mach: Machine = REFERENCE) -> List[str]:
     for shape in ("square", "rankk", "fixk"):
         res = generate_result_mat(coef, shape, mach)
         for i, (L, pack) in enumerate([(1, "abc"), (1, "ab"), (1, "naive"), (2, "abc"), (2, "ab"), (2, "naive")]):
-            path = os.path.join(out_dir, f"model_{shape}_{L}level_{pack}_1core.csv")
+            path = os.path.join(out_dir, f"model_{shape}_{L}level_{pack}_{csv_suffix(mach)}.csv")
             off = i * n
             with open(path, "w") as fh:
                 fh.write(HEADER + "\n")
@@ -189,6 +207,58 @@ def csv_files(out_dir: str, mach: Machine = REFERENCE) -> List[str]:
     return written
 
 
+def check_against_sweep(rows: Sequence[Dict[str, float]], mach: Machine = MI355X_F64) -> Dict[str, dict]:
+    """Model against measured sweep points (rows of a sweep CSV, sweep.py):
+    per variant '<alg>-<L>_<pack>' whose algorithm is in the registry, the
+    median and range of measured / modelled effective GFLOPS, and the share
+    of points where model and measurement agree on whether the variant beats
+    the plain GEMM."""
+    out: Dict[str, dict] = {}
+    if not rows:
+        return out
+    labels = [c for c in rows[0] if c not in ("m", "n", "k", "gemm") and "/" not in c]
+    for lab in labels:
+        name, rest = lab.split("-", 1)
+        level_s, pack = rest.split("_", 1)
+        if name not in alg_dic or pack not in PACKS:
+            continue
+        path, dims = alg_dic[name]
+        alg = compose([load_algorithm(path, dims, name)] * int(level_s))
+        coef = model_row(alg, pack)
+        ratios, agree = [], 0
+        for r in rows:
+            m, n, k = int(r["m"]), int(r["n"]), int(r["k"])
+            mod = get_model_gflops(m, n, k, dims, int(level_s), coef, mach)
+            mod_gemm = get_model_gflops(m, n, k, (1, 1, 1), 1, GEMM_ROW, mach)
+            ratios.append(r[lab] / mod)
+            agree += (r[lab] > r["gemm"]) == (mod > mod_gemm)
+        ratios.sort()
+        out[lab] = {"measured_over_model_median": round(ratios[len(ratios) // 2], 3),
+                    "measured_over_model_min": round(ratios[0], 3),
+                    "measured_over_model_max": round(ratios[-1], 3),
+                    "winner_agreement": round(agree / len(rows), 3), "points": len(rows)}
+    return out
+
+
+def main(argv=None) -> None:
+    """python model.py [out_dir] [--machine NAME] [--check sweep.csv ...]:
+    write the 18 CSVs of a machine (the reference's model_gen.py main), and
+    optionally compare the model with measured sweep CSVs."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="performance model CSVs (meta/model_gen.py)")
+    ap.add_argument("out_dir", nargs="?", default="model_csv")
+    ap.add_argument("--machine", default=REFERENCE.name, choices=sorted(MACHINES))
+    ap.add_argument("--check", nargs="*", default=[], help="sweep CSVs (sweep.py) to check the model against")
+    a = ap.parse_args(argv)
+    mach = MACHINES[a.machine]
+    print("\n".join(csv_files(a.out_dir, mach)))
+    for path in a.check:
+        import csv
+        with open(path) as fh:
+            rows = [{k: float(v) for k, v in row.items()} for row in csv.DictReader(fh)]
+        print(json.dumps({"sweep": path, "machine": mach.name, "check": check_against_sweep(rows, mach)}))
+
+
 if __name__ == "__main__":
-    import sys
-    print("\n".join(csv_files(sys.argv[1] if len(sys.argv) > 1 else "model_csv")))
+    main()

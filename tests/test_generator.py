@@ -96,3 +96,32 @@ def test_job_driver_prints_reference_line(job):
     assert float(cols[3]) > 0 and float(cols[4]) > 0
     res = subprocess.run([exe, "10"], capture_output=True, text=True, timeout=60)
     assert "require 3 arguments" in res.stdout
+
+
+@pytest.mark.gpu
+def test_custom_table_job_equals_runtime_plan(fmm, rt):
+    """The job library abc_gen.py built from table files (Strassen then
+    smirnov333-23-139) gives the runtime plan's result for the same composition
+    (registry names 222, 333) bit for bit, on float inputs, through the
+    reference's host entry and the device entry."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from __graft_entry__ import CUSTOM_DIR
+    L = rt.load_library(os.path.join(CUSTOM_DIR, "libfmm_custom.so"), has_job=True)
+    assert L.job_pack == "abc"
+    plan = rt.Plan(fmm.build_algorithm(["222", "333"], [1, 1]))
+    rng = np.random.default_rng(6)
+    m, n, k = 6 * 8 * 9 + 5, 6 * 4 * 13 + 3, 6 * 40 + 2
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+    C = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    host = C.copy(order="F")
+    dp = ct.POINTER(ct.c_double)
+    L.bl_dgemm_strassen_abc(m, n, k, A.ctypes.data_as(dp), m, B.ctypes.data_as(dp), k, host.ctypes.data_as(dp), m)
+    dA, dB, dC, dR = (torch.from_numpy(X.T).to("cuda").t() for X in (A, B, C, C))
+    assert L.fmm_job_abc(m, n, k, dA.data_ptr(), m, dB.data_ptr(), k, dC.data_ptr(), m, None, 0, None) == 0
+    plan.abc(dA, dB, dR)
+    torch.cuda.synchronize()
+    assert np.array_equal(host, dR.cpu().numpy())
+    assert torch.equal(dC, dR)
