@@ -181,3 +181,31 @@ def test_config1_strassen_2048(torch, fmm, rt, oracle):
     ref = C.copy(order="F")
     oracle.fmm(alg, A, B, ref, "abc")
     assert float(np.abs(got - ref).max()) <= 2 * tol_for(1) * 2.0 ** -53 * k
+
+
+@pytest.mark.parametrize("handover", ["0", "1"])
+def test_ordered_launch_both_handover_modes(torch, fmm, rt, oracle, handover, monkeypatch):
+    """The ordered launches give the same results with the XCD-local C
+    hand-over (release = wait for the writer's stores in the shared L2) and
+    with the agent-scope release everywhere (FMM_LOCAL_HANDOVER=0): integer
+    cases bit-exact, float cases bitwise equal between the modes.  Shapes with
+    many products per C tile and more units than workgroups."""
+    monkeypatch.setenv("FMM_LOCAL_HANDOVER", handover)
+    rng = np.random.default_rng(31)
+    for spec in ((["222", "323"], [1, 1]), (["333"], [1])):
+        alg = fmm.build_algorithm(*spec)
+        plan = rt.Plan(alg)
+        Mx, Kx, Nx = alg.dims
+        m, n, k = Mx * 8 * 40 + 3, Nx * 4 * 70 + 1, Kx * 90 + 2
+        A, B, C = int_mats(rng, m, n, k)
+        got = run_abc(torch, rt, plan, A, B, C)
+        assert np.array_equal(got, C + A @ B), (spec, handover)
+        Af = rng.uniform(-1, 1, (m, k))
+        Bf = rng.uniform(-1, 1, (k, n))
+        Cf = rng.uniform(-1, 1, (m, n))
+        f_mode = run_abc(torch, rt, plan, Af, Bf, Cf)
+        monkeypatch.setenv("FMM_LOCAL_HANDOVER", "1" if handover == "0" else "0")
+        f_other = run_abc(torch, rt, plan, Af, Bf, Cf)
+        monkeypatch.setenv("FMM_LOCAL_HANDOVER", handover)
+        assert np.array_equal(f_mode, f_other), spec
+        plan.close()
