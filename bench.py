@@ -56,7 +56,7 @@ CONFIG2 = ["333L1_sq9600", "222x323_sq9600", "333L1_rankk14400x480", "222x323_ra
 # shapes where the FMM margin over GEMM is clear of run-to-run noise: a large
 # square (two Strassen levels) and the deep end of the rank-k sweep
 MARGIN = {"222L2_sq16384": (["222"], [2], 16384, 16384, 16384),
-          "222L1_rankk14400x7200": (["222"], [1], 14400, 14400, 7200)}
+          "323L1_rankk14400x9600": (["323"], [1], 14400, 14400, 9600)}
 
 
 def parse():
