@@ -151,8 +151,12 @@ constexpr int FN = WTN / 16;           // 2 MFMA blocks across
 // A slice (BM x BK, column-major source): thread t owns rows 2*(t%64)+{0,1}
 // of columns t/64 + A_CPP*i: each wave reads one contiguous 1 KiB column run.
 constexpr int A_CPP = NT / 64;         // columns covered per pass
-// B slice (BK x BN, column-major source): B_TPC threads per column, each
-// with BK/B_TPC consecutive k.
+// B slice (BK x BN, column-major source): BK/2 consecutive lanes cover the
+// BK-deep run of one column (16 bytes each), so every load instruction reads
+// whole cache lines; a thread takes the same k pair of B_IT columns.
+#ifndef FMM_BLINE
+#define FMM_BLINE 1
+#endif
 constexpr int B_TPC = NT / BN;
 
 // Slice depth BK is a template parameter: 32 suits products whose operand
@@ -165,8 +169,13 @@ struct TileK {
   static constexpr int A_IT = BK / A_CPP;     // A passes per slice
   static constexpr int AE = 2 * A_IT;         // A values per thread per slice
   static constexpr int B_KPT = BK / B_TPC;    // B k-run per thread
+  static constexpr int B_LPC = BK / 2;        // lanes per column (line layout)
+  static constexpr int B_CPP = NT / B_LPC;    // columns per pass (line layout)
+  static constexpr int B_IT = BN / B_CPP;     // passes (line layout)
+  static constexpr bool LINE = FMM_BLINE && BK_ <= 16;
   static constexpr int BE = B_KPT;            // B values per thread per slice
   static_assert(BM == 128 && A_CPP * A_IT == BK && B_TPC * B_KPT == BK, "staging layout");
+  static_assert(B_CPP * B_IT == BN && 2 * B_IT == BE, "B line layout");
 };
 static_assert(WAVES_M * WAVES_N * 64 == NT, "wave layout");
 
@@ -299,11 +308,20 @@ __device__ __forceinline__ void load_raw(const Job& J, const Srcs<CH>& S, int kb
       }
       if (t < S.nb) {
         gcdouble* base = S.b[t] + kb;
+        if constexpr (TileK<BK>::LINE) {
 #pragma unroll
-        for (int e = 0; e < B_KPT; e += 2) {
-          const f64x2 v = *(gcf64x2*)(base + e + g.boff);
-          r.b[t][e] = v.x;
-          r.b[t][e + 1] = v.y;
+          for (int i = 0; i < TileK<BK>::B_IT; ++i) {
+            const f64x2 v = *(gcf64x2*)(base + (long long)(TileK<BK>::B_CPP * i) * J.ldb + g.boff);
+            r.b[t][2 * i] = v.x;
+            r.b[t][2 * i + 1] = v.y;
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < B_KPT; e += 2) {
+            const f64x2 v = *(gcf64x2*)(base + e + g.boff);
+            r.b[t][e] = v.x;
+            r.b[t][e + 1] = v.y;
+          }
         }
       }
     }
@@ -332,16 +350,31 @@ __device__ __forceinline__ void load_raw(const Job& J, const Srcs<CH>& S, int kb
       }
       if (t < S.nb) {
         gcdouble* base = S.b[t] + kb;
+        if constexpr (TileK<BK>::LINE) {
 #pragma unroll
-        for (int e = 0; e < B_KPT; e += 2) {
-          f64x2 v = {0.0, 0.0};
-          if (e + 1 < krem) {
-            v = *(gcf64x2*)(base + e + g.boff);
-          } else if (e < krem) {
-            v.x = base[e + g.boff];
+          for (int i = 0; i < TileK<BK>::B_IT; ++i) {
+            gcdouble* p = base + (long long)(TileK<BK>::B_CPP * i) * J.ldb + g.boff;
+            f64x2 v = {0.0, 0.0};
+            if (1 < krem) {
+              v = *(gcf64x2*)p;
+            } else if (0 < krem) {
+              v.x = p[0];
+            }
+            r.b[t][2 * i] = v.x;
+            r.b[t][2 * i + 1] = v.y;
           }
-          r.b[t][e] = v.x;
-          r.b[t][e + 1] = v.y;
+        } else {
+#pragma unroll
+          for (int e = 0; e < B_KPT; e += 2) {
+            f64x2 v = {0.0, 0.0};
+            if (e + 1 < krem) {
+              v = *(gcf64x2*)(base + e + g.boff);
+            } else if (e < krem) {
+              v.x = base[e + g.boff];
+            }
+            r.b[t][e] = v.x;
+            r.b[t][e + 1] = v.y;
+          }
         }
       }
     }
@@ -365,8 +398,18 @@ __device__ __forceinline__ void load_raw(const Job& J, const Srcs<CH>& S, int kb
     }
     if (t < S.nb) {
       gcdouble* p = S.b[t] + kb + g.boff;
+      if constexpr (TileK<BK>::LINE) {
 #pragma unroll
-      for (int e = 0; e < B_KPT; ++e) r.b[t][e] = (bcok && e < krem) ? p[e] : 0.0;
+        for (int i = 0; i < TileK<BK>::B_IT; ++i) {
+          const bool cok = g.col0 + g.bcol + TileK<BK>::B_CPP * i < J.ns;
+          gcdouble* q = p + (long long)(TileK<BK>::B_CPP * i) * J.ldb;
+          r.b[t][2 * i] = (cok && 0 < krem) ? q[0] : 0.0;
+          r.b[t][2 * i + 1] = (cok && 1 < krem) ? q[1] : 0.0;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < B_KPT; ++e) r.b[t][e] = (bcok && e < krem) ? p[e] : 0.0;
+      }
     }
   }
 }
@@ -418,8 +461,16 @@ __device__ __forceinline__ void store_slice(double (*As)[BM], double (*Bs)[TileK
     v.y = sa[2 * i + 1];
     *reinterpret_cast<double2*>(&As[g.acol + A_CPP * i][g.arow]) = v;
   }
+  if constexpr (TileK<BK>::LINE) {
 #pragma unroll
-  for (int e = 0; e < B_KPT; ++e) Bs[g.bcol][g.bk + e] = sb[e];
+    for (int i = 0; i < TileK<BK>::B_IT; ++i) {
+      Bs[g.bcol + TileK<BK>::B_CPP * i][g.bk] = sb[2 * i];
+      Bs[g.bcol + TileK<BK>::B_CPP * i][g.bk + 1] = sb[2 * i + 1];
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < B_KPT; ++e) Bs[g.bcol][g.bk + e] = sb[e];
+  }
 }
 
 // Operand sums of slice kb (chunk 0 already loaded into r) stored to LDS.
@@ -460,11 +511,17 @@ __device__ __forceinline__ void pack_slice(const Job& J, const Prod& P, const Sr
 // few free registers for either and run faster without both (ABC 333: 49.6
 // -> 51.4 TFLOP/s), but gain from the raised priority (s_setprio: 49.6 ->
 // 51.5).
+#ifndef FMM_ABC_SCHED
+#define FMM_ABC_SCHED 0
+#endif
 template <int CH>
 struct F64Sched {
-  static constexpr bool frag_pipe = CH == 1;
-  static constexpr bool pingpong = CH == 1;
-  static constexpr bool setprio = CH != 1;
+#ifndef FMM_ABC_FP
+#define FMM_ABC_FP 0
+#endif
+  static constexpr bool frag_pipe = CH == 1 || FMM_ABC_FP != 0;
+  static constexpr bool pingpong = CH == 1 || FMM_ABC_SCHED != 0;
+  static constexpr bool setprio = CH != 1 && FMM_ABC_SCHED != 2;
 };
 
 template <int BK, int KK0, int KK1, bool FRAG_PIPE, bool SETPRIO>
@@ -672,6 +729,269 @@ __device__ __forceinline__ void contract(const Job& J, const Prod& P, const Unit
   }
 }
 
+// --- operand sums in staggered rounds (CH == kRounds) ------------------------
+//
+// The 32-deep single-term pipeline (fragments of step kk+1 read during the
+// MFMAs of step kk) with operand sums of up to kRoundTerms terms per operand:
+// the sums of slice st+1 are built during the MFMAs of slice st one term at a
+// time.  Round t (term t of A and of B) is loaded into the one raw register
+// set, then folded into the running sums after two further k-steps:
+//
+//   end of slice st-1: load round 0 of slice st+1
+//   after k-step 1:    s = u0*x0            ; load round 1
+//   after k-step 3:    s = fma(u1, x1, s)   ; load round 2
+//   after k-step 5:    s = fma(u2, x2, s)   ; load round 3
+//   after k-step 7:    s = fma(u3, x3, s)
+//   end of slice st:   store s to LDS       ; load round 0 of slice st+2
+//
+// so only one term's raw values are live at a time (16 instead of 64 FP64
+// values per thread for 4 terms) and the 32-deep slice (half the barriers
+// of a 16-deep one) fits the register file.  The sums are formed in the same
+// order as in the chunked kernels (s = u0*x0, then s = fma(u_t, x_t, s)), so
+// the results are bit-identical.  Terms beyond kRoundTerms (tables with more
+// than 4 terms per operand) are loaded and folded at the end of the slice
+// without prefetch.
+constexpr int kRounds = 0;
+constexpr int kRoundTerms = 4;
+
+__device__ __forceinline__ Srcs<1> round_src(const Srcs<kRoundTerms>& S, int t) {
+  Srcs<1> R;
+  R.na = t < S.na ? 1 : 0;
+  R.nb = t < S.nb ? 1 : 0;
+  R.a[0] = S.a[t];
+  R.b[0] = S.b[t];
+  R.u[0] = S.u[t];
+  R.v[0] = S.v[t];
+  return R;
+}
+
+// MFMAs of one whole slice with fragment pipelining, calling hook(kk) after
+// the MFMAs of k-step kk are issued (kk is a constant after unrolling).
+template <int BK, class Hook>
+__device__ __forceinline__ void mfma_slice_hooked(const double (*As)[BM], const double (*Bs)[TileK<BK>::BKP], int lane,
+                                                  int wm, int wn, f64x4 (&acc)[FM][FN], Hook&& hook) {
+  constexpr int KK = BK / 4;
+  const int l16 = lane & 15, lk = lane >> 4;
+  double a[2][FM], b[2][FN];
+#pragma unroll
+  for (int mi = 0; mi < FM; ++mi) a[0][mi] = As[lk][wm * WTM + mi * 16 + l16];
+#pragma unroll
+  for (int ni = 0; ni < FN; ++ni) b[0][ni] = Bs[wn * WTN + ni * 16 + l16][lk];
+#pragma unroll
+  for (int kk = 0; kk < KK; ++kk) {
+    const int c = kk & 1;
+    if (kk + 1 < KK) {
+#pragma unroll
+      for (int mi = 0; mi < FM; ++mi) a[c ^ 1][mi] = As[(kk + 1) * 4 + lk][wm * WTM + mi * 16 + l16];
+#pragma unroll
+      for (int ni = 0; ni < FN; ++ni) b[c ^ 1][ni] = Bs[wn * WTN + ni * 16 + l16][(kk + 1) * 4 + lk];
+    }
+#pragma unroll
+    for (int mi = 0; mi < FM; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < FN; ++ni)
+        acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[c][ni], a[c][mi], acc[mi][ni], 0, 0, 0);
+    hook(kk);
+  }
+}
+
+// Rounds beyond kRoundTerms of slice kb, folded into s without prefetch.
+template <int BK>
+__device__ __forceinline__ void extra_rounds(const Job& J, const Prod& P, int kb, const UnitGeo& g, Raw<BK, 1>& r,
+                                             Slice<BK>& s) {
+  const int nr = P.na > P.nb ? P.na : P.nb;
+  for (int t = kRoundTerms; t < nr; ++t) {
+    Srcs<1> S;
+    make_srcs<1>(J, P.a0 + t, P.na - t, P.b0 + t, P.nb - t, S);
+    load_raw<BK, 1>(J, S, kb, g, r);
+    accumulate<BK, 1>(S, false, r, s);
+  }
+}
+
+// Unpredicated loads of one operand term's slice kb (interior tile, whole
+// slice inside the leaf depth), addressed as a uniform 64-bit base (the term's
+// block, the tile's first row or column and kb: scalar arithmetic) plus a
+// per-thread 32-bit byte offset that is the same for every term, so the
+// loads take the SGPR-base + VGPR-offset form and no per-term address lives
+// in vector registers.
+struct ThreadOff {
+  long long lda;   // uniform copies of the job's leading dimensions
+  long long ldb;
+  int row0, col0;  // the tile's first row / column (uniform)
+  unsigned a;      // (arow + acol * lda) * 8
+  unsigned astep;  // A_CPP * lda * 8 (uniform)
+  unsigned b;      // (bcol * ldb + bk) * 8
+};
+
+__device__ __forceinline__ ThreadOff thread_off(const Job& J, const UnitGeo& g) {
+  ThreadOff o;
+  o.lda = uni64(J.lda);
+  o.ldb = uni64(J.ldb);
+  o.row0 = __builtin_amdgcn_readfirstlane(g.row0);
+  o.col0 = __builtin_amdgcn_readfirstlane(g.col0);
+  // Rows and columns past the leaf block (edge tiles) are clamped to the
+  // last row pair / column: they load real, finite values, and the MFMA
+  // results of those rows and columns are never stored (scatter_c bounds).
+  // Needs an even leaf row count, so the clamped row pair stays 16-byte
+  // aligned; the depth must be a multiple of the slice (no k predicate).
+  const int arow = min(g.row0 + g.arow, J.ms - 2) - g.row0;
+  const int bcol = min(g.col0 + g.bcol, J.ns - 1) - g.col0;
+  o.a = (unsigned)((arow + (long long)g.acol * o.lda) * 8);
+  o.astep = (unsigned)__builtin_amdgcn_readfirstlane((int)((long long)A_CPP * o.lda * 8));
+  o.b = (unsigned)(((long long)bcol * o.ldb + g.bk) * 8);
+  return o;
+}
+
+typedef __attribute__((address_space(1))) const char gcchar;
+
+template <int BK>
+__device__ __forceinline__ void load_a_full(gcdouble* block, int kb, const ThreadOff& o,
+                                            double (&ra)[TileK<BK>::AE]) {
+  gcchar* base = (gcchar*)(block + o.row0 + (long long)kb * o.lda);
+#pragma unroll
+  for (int i = 0; i < TileK<BK>::A_IT; ++i) {
+    const f64x2 v = *(gcf64x2*)(base + (o.a + (unsigned)i * o.astep));
+    ra[2 * i] = v.x;
+    ra[2 * i + 1] = v.y;
+  }
+}
+template <int BK>
+__device__ __forceinline__ void load_b_full(gcdouble* block, int kb, const ThreadOff& o,
+                                            double (&rb)[TileK<BK>::BE]) {
+  static_assert(!TileK<BK>::LINE, "k-run layout");
+  gcchar* base = (gcchar*)(block + (long long)o.col0 * o.ldb + kb);
+#pragma unroll
+  for (int e = 0; e < TileK<BK>::B_KPT; e += 2) {
+    const f64x2 v = *(gcf64x2*)(base + (o.b + (unsigned)(e * 8)));
+    rb[e] = v.x;
+    rb[e + 1] = v.y;
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void fold(double u, bool first, const double (&x)[N], double (&acc)[N]) {
+#pragma unroll
+  for (int e = 0; e < N; ++e) acc[e] = first ? u * x[e] : __builtin_fma(u, x[e], acc[e]);
+}
+
+// contract_rounds for a unit whose depth is a multiple of BK, with 16-byte
+// operand access and an even leaf row count, whose product has NA A terms
+// and NB B terms (both <= kRoundTerms): every
+// load, fold and wait is fixed at compile time, with no branch inside a
+// slice but the one for the last slice, so each fold waits only for its own
+// round's loads (with run-time term counts the waits degrade to vmcnt(0)
+// after every round and the loads of the next round are waited for too).
+template <int BK, int NA, int NB>
+__device__ __forceinline__ void contract_rounds_fixed(const Job& J, const Srcs<kRoundTerms>& S4, int nsteps,
+                                                      const UnitGeo& g, double (*As)[BK][BM],
+                                                      double (*Bs)[BN][TileK<BK>::BKP], int lane, int wm, int wn,
+                                                      f64x4 (&acc)[FM][FN]) {
+  constexpr int NR = NA > NB ? NA : NB;
+  static_assert(NA >= 1 && NB >= 1 && NR <= kRoundTerms && BK / 4 >= 2 * NR, "round layout");
+  double ra[TileK<BK>::AE], rb[TileK<BK>::BE];
+  Slice<BK> s;
+  const ThreadOff o = thread_off(J, g);
+  const int klast = (nsteps - 1) * BK;   // the next slice's loads are clamped here: the one
+                                         // past-the-end round 0 re-reads the last slice, unused
+  auto issue = [&](int t, int kb) {
+    if (t < NA) load_a_full<BK>(S4.a[t], kb, o, ra);
+    if (t < NB) load_b_full<BK>(S4.b[t], kb, o, rb);
+  };
+  auto consume = [&](int t) {
+    if (t < NA) fold(S4.u[t], t == 0, ra, s.a);
+    if (t < NB) fold(S4.v[t], t == 0, rb, s.b);
+  };
+#pragma unroll
+  for (int t = 0; t < NR; ++t) {
+    issue(t, 0);
+    consume(t);
+  }
+  store_slice<BK>(As[0], Bs[0], g, s.a, s.b);
+  issue(0, min(BK, klast));
+  __syncthreads();
+  int st = 0;
+  for (; st + 1 < nsteps; ++st) {
+    const int cur = st & 1;
+    const int kb1 = (st + 1) * BK;
+    mfma_slice_hooked<BK>(As[cur], Bs[cur], lane, wm, wn, acc, [&](int kk) {
+      const int t = kk >> 1;
+      if ((kk & 1) && t < NR) {
+        // keep the machine scheduler from hoisting later rounds' loads
+        // (each hoisted round would hold another raw register set)
+        __builtin_amdgcn_sched_barrier(0);
+        consume(t);
+        __builtin_amdgcn_sched_barrier(0);
+        if (t + 1 < NR) issue(t + 1, kb1);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    });
+    store_slice<BK>(As[cur ^ 1], Bs[cur ^ 1], g, s.a, s.b);
+    issue(0, min(kb1 + BK, klast));
+    __syncthreads();
+  }
+  mfma_slice_hooked<BK>(As[st & 1], Bs[st & 1], lane, wm, wn, acc, [](int) {});
+  __syncthreads();
+}
+
+template <int BK>
+__device__ __forceinline__ void contract_rounds(const Job& J, const Prod& P, const UnitGeo& g, double (*As)[BK][BM],
+                                                double (*Bs)[BN][TileK<BK>::BKP], int lane, int wm, int wn,
+                                                f64x4 (&acc)[FM][FN]) {
+  static_assert(BK / 4 >= 2 * kRoundTerms, "one round per two k-steps");
+  const int nsteps = (g.depth + BK - 1) / BK;
+  if (!(P.na > 0 && P.nb > 0 && nsteps > 0)) return;
+  Srcs<kRoundTerms> S4;
+  make_srcs<kRoundTerms>(J, P.a0, P.na, P.b0, P.nb, S4);
+  const int nr = __builtin_amdgcn_readfirstlane(P.na > P.nb ? P.na : P.nb);
+  if ((J.flags & 3) == 3 && (J.ms & 1) == 0 && g.depth % BK == 0 && nr <= kRoundTerms) {
+    switch (P.na * 8 + P.nb) {
+#define FMM_ROUNDS_CASE(a, b) \
+  case (a) * 8 + (b): contract_rounds_fixed<BK, a, b>(J, S4, nsteps, g, As, Bs, lane, wm, wn, acc); return;
+      FMM_ROUNDS_CASE(1, 1) FMM_ROUNDS_CASE(1, 2) FMM_ROUNDS_CASE(1, 3) FMM_ROUNDS_CASE(1, 4)
+      FMM_ROUNDS_CASE(2, 1) FMM_ROUNDS_CASE(2, 2) FMM_ROUNDS_CASE(2, 3) FMM_ROUNDS_CASE(2, 4)
+      FMM_ROUNDS_CASE(3, 1) FMM_ROUNDS_CASE(3, 2) FMM_ROUNDS_CASE(3, 3) FMM_ROUNDS_CASE(3, 4)
+      FMM_ROUNDS_CASE(4, 1) FMM_ROUNDS_CASE(4, 2) FMM_ROUNDS_CASE(4, 3) FMM_ROUNDS_CASE(4, 4)
+#undef FMM_ROUNDS_CASE
+      default: break;
+    }
+  }
+  Raw<BK, 1> r;
+  Slice<BK> s;
+  // slice 0, without overlap
+#pragma unroll
+  for (int t = 0; t < kRoundTerms; ++t) {
+    if (t < nr) {
+      const Srcs<1> R = round_src(S4, t);
+      load_raw<BK, 1>(J, R, 0, g, r);
+      accumulate<BK, 1>(R, t == 0, r, s);
+    }
+  }
+  if (nr > kRoundTerms) extra_rounds<BK>(J, P, 0, g, r, s);
+  store_slice<BK>(As[0], Bs[0], g, s.a, s.b);
+  if (nsteps > 1) load_raw<BK, 1>(J, round_src(S4, 0), BK, g, r);
+  __syncthreads();
+  for (int st = 0; st < nsteps; ++st) {
+    const int cur = st & 1;
+    const bool next = st + 1 < nsteps;
+    const int kb1 = (st + 1) * BK;
+    mfma_slice_hooked<BK>(As[cur], Bs[cur], lane, wm, wn, acc, [&](int kk) {
+      if ((kk & 1) && next) {
+        const int t = kk >> 1;
+        if (t < kRoundTerms && t < nr) {
+          accumulate<BK, 1>(round_src(S4, t), t == 0, r, s);
+          if (t + 1 < kRoundTerms && t + 1 < nr) load_raw<BK, 1>(J, round_src(S4, t + 1), kb1, g, r);
+        }
+      }
+    });
+    if (next) {
+      if (nr > kRoundTerms) extra_rounds<BK>(J, P, kb1, g, r, s);
+      store_slice<BK>(As[cur ^ 1], Bs[cur ^ 1], g, s.a, s.b);
+      if (st + 2 < nsteps) load_raw<BK, 1>(J, round_src(S4, 0), kb1 + BK, g, r);
+    }
+    __syncthreads();
+  }
+}
+
 // Map queue-local unit index i of queue x to (job, product group, tile).
 // Each job's tiles are split into nq contiguous chunks; queue x runs, job by
 // job, all product groups over its chunk (groups in order, tiles innermost),
@@ -802,8 +1122,13 @@ __global__ void __launch_bounds__(NT) abc_f64_kernel(Launch L) {
     g.col0 = tni * BN;
     g.arow = 2 * (tid & 63);
     g.acol = tid >> 6;
-    g.bcol = tid / B_TPC;
-    g.bk = B_KPT * (tid % B_TPC);
+    if constexpr (TileK<BK>::LINE) {
+      g.bcol = tid / TileK<BK>::B_LPC;
+      g.bk = 2 * (tid % TileK<BK>::B_LPC);
+    } else {
+      g.bcol = tid / B_TPC;
+      g.bk = B_KPT * (tid % B_TPC);
+    }
     g.interior = (J.flags & 3) == 3 && g.row0 + BM <= J.ms && g.col0 + BN <= J.ns;
     g.aoff = g.row0 + g.arow + (long long)g.acol * J.lda;
     g.boff = (long long)(g.col0 + g.bcol) * J.ldb + g.bk;
@@ -832,7 +1157,11 @@ __global__ void __launch_bounds__(NT) abc_f64_kernel(Launch L) {
 #pragma unroll
         for (int ni = 0; ni < FN; ++ni) acc[mi][ni] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-      contract<BK, CH>(J, P, g, As, Bs, lane, wm, wn, wave >> 2, acc);
+      if constexpr (CH == kRounds) {
+        contract_rounds<BK>(J, P, g, As, Bs, lane, wm, wn, acc);
+      } else {
+        contract<BK, CH>(J, P, g, As, Bs, lane, wm, wn, wave >> 2, acc);
+      }
 
       if (ordered && q > 0 && qq == q0) wait_turn(ctr, q, L);
       scatter_c(J, P, g.row0, g.col0, lane, wm, wn, acc);

@@ -783,8 +783,28 @@ inline int launch_jobs_k(JobList JL, int* ws, hipStream_t stream) {
 // (Running the > 2-term products on the 32-deep 2-term kernel too, their
 // further terms loaded without prefetch, measured 2-5% slower at 9600^3 for
 // <3,3,3>, <3,2,3> and <2,2,2>x<3,2,3>.)
+// The staggered-round kernel (32-deep slices, operand sums built one term at
+// a time during the MFMAs, see contract_rounds) runs a launch when every
+// product has at most kRoundTerms terms per operand and every FMM core job
+// takes its compile-time-shaped path (16-byte operand access, even leaf row
+// count, leaf depth a multiple of 32); the chunked kernels serve the rest.
+// FMM_ABC_ROUNDS=0 selects the chunked kernels always (the two agree bit for
+// bit).  Measured on the 333 L1 9600^3 ABC launch: 32.5 -> 27.3 ms.
+inline bool rounds_enabled() {
+  const char* e = getenv("FMM_ABC_ROUNDS");
+  return !(e && e[0] == '0');
+}
+
+inline bool rounds_fit(const JobList& JL) {
+  for (const Job& j : JL.jobs)
+    if (j.nprod > 1 && !((j.flags & 3) == 3 && j.ms % 2 == 0 && j.ks % 32 == 0)) return false;
+  return true;
+}
+
 inline int launch_jobs(const JobList& JL, int max_terms, int* ws, hipStream_t stream) {
   if (max_terms <= 1) return launch_jobs_k<32, 1>(JL, ws, stream);
+  if (max_terms <= kRoundTerms && rounds_enabled() && rounds_fit(JL))
+    return launch_jobs_k<32, kRounds>(JL, ws, stream);
   if (max_terms <= 2) return launch_jobs_k<32, 2>(JL, ws, stream);
   return launch_jobs_k<16, 4>(JL, ws, stream);
 }

@@ -115,6 +115,48 @@ def test_packs_bitwise_equal_abc(torch, fmm, rt, spec):
             assert np.array_equal(got, base), (spec, pack, m, n, k, float(np.abs(got - base).max()))
 
 
+@contextlib.contextmanager
+def env(name, value):
+    old = os.environ.get(name)
+    os.environ[name] = value
+    try:
+        yield
+    finally:
+        if old is None:
+            os.environ.pop(name, None)
+        else:
+            os.environ[name] = old
+
+
+@pytest.mark.parametrize("spec", SPECS, ids=IDS)
+def test_abc_round_kernel_bitwise(torch, fmm, rt, oracle, spec):
+    """The staggered-round ABC kernel (leaf depth a multiple of 32) against
+    the chunked ABC kernels (FMM_ABC_ROUNDS=0): bitwise equal on random
+    inputs, with edge tiles (leaf rows / columns not multiples of 128, whose
+    out-of-range rows and columns are loaded clamped) and k-fringes; exact on
+    integer inputs against numpy and the oracle."""
+    alg = fmm.build_algorithm(*spec)
+    plan = rt.Plan(alg)
+    Mx, Kx, Nx = alg.dims
+    rng = np.random.default_rng(21)
+    for (m, n, k) in [(Mx * 8 * 20, Nx * 4 * 40, Kx * 64),
+                      (Mx * 8 * 17 + 6, Nx * 4 * 29 + 1, Kx * 96 + 2),
+                      (Mx * 136, Nx * 132, Kx * 32)]:
+        A = rng.uniform(-1, 1, (m, k))
+        B = rng.uniform(-1, 1, (k, n))
+        C = rng.uniform(-1, 1, (m, n))
+        got = run(torch, rt, plan, "abc", A, B, C)
+        with env("FMM_ABC_ROUNDS", "0"):
+            want = run(torch, rt, plan, "abc", A, B, C)
+        assert np.array_equal(got, want), (spec, m, n, k, float(np.abs(got - want).max()))
+        Ai, Bi, Ci = int_mats(rng, m, n, k)
+        goti = run(torch, rt, plan, "abc", Ai, Bi, Ci)
+        assert np.array_equal(goti, Ci + Ai @ Bi), (spec, m, n, k)
+        ref = Ci.copy(order="F")
+        oracle.fmm(alg, Ai, Bi, ref, "abc")
+        assert np.array_equal(goti, ref)
+
+
 @pytest.mark.parametrize("pack", PACKS)
 def test_pack_strided_leading_dims(torch, fmm, rt, pack):
     """Odd leading dimensions and odd leaf depth take the element-wise paths
