@@ -821,6 +821,9 @@ struct ThreadOff {
   unsigned a;      // (arow + acol * lda) * 8
   unsigned astep;  // A_CPP * lda * 8 (uniform)
   unsigned b;      // (bcol * ldb + bk) * 8
+  unsigned a0;     // arow * 8 (the row part of a)
+  unsigned b0;     // bcol * ldb * 8 (the column part of b)
+  int acol, bk;    // this thread's first A column / B k within a slice
 };
 
 __device__ __forceinline__ ThreadOff thread_off(const Job& J, const UnitGeo& g) {
@@ -839,6 +842,10 @@ __device__ __forceinline__ ThreadOff thread_off(const Job& J, const UnitGeo& g) 
   o.a = (unsigned)((arow + (long long)g.acol * o.lda) * 8);
   o.astep = (unsigned)__builtin_amdgcn_readfirstlane((int)((long long)A_CPP * o.lda * 8));
   o.b = (unsigned)(((long long)bcol * o.ldb + g.bk) * 8);
+  o.a0 = (unsigned)(arow * 8);
+  o.b0 = (unsigned)((long long)bcol * o.ldb * 8);
+  o.acol = g.acol;
+  o.bk = g.bk;
   return o;
 }
 
@@ -868,22 +875,54 @@ __device__ __forceinline__ void load_b_full(gcdouble* block, int kb, const Threa
   }
 }
 
+// The partial last slice kb (depth - kb < BK): columns / k at or past the
+// depth read the last valid column / k pair instead and are zeroed.  The
+// depth is even (16-byte B access needs an even leaf depth), so a B k pair is
+// valid or invalid as a whole.
+template <int BK>
+__device__ __forceinline__ void load_a_tail(gcdouble* block, int kb, int depth, const ThreadOff& o,
+                                            double (&ra)[TileK<BK>::AE]) {
+  gcchar* base = (gcchar*)(block + o.row0 + (long long)kb * o.lda);
+  const unsigned last = o.a0 + (unsigned)((long long)(depth - 1 - kb) * o.lda * 8);
+#pragma unroll
+  for (int i = 0; i < TileK<BK>::A_IT; ++i) {
+    const bool ok = kb + o.acol + A_CPP * i < depth;
+    const f64x2 v = *(gcf64x2*)(base + (ok ? o.a + (unsigned)i * o.astep : last));
+    ra[2 * i] = ok ? v.x : 0.0;
+    ra[2 * i + 1] = ok ? v.y : 0.0;
+  }
+}
+template <int BK>
+__device__ __forceinline__ void load_b_tail(gcdouble* block, int kb, int depth, const ThreadOff& o,
+                                            double (&rb)[TileK<BK>::BE]) {
+  gcchar* base = (gcchar*)(block + (long long)o.col0 * o.ldb + kb);
+  const unsigned last = o.b0 + (unsigned)((depth - 2 - kb) * 8);
+#pragma unroll
+  for (int e = 0; e < TileK<BK>::B_KPT; e += 2) {
+    const bool ok = kb + o.bk + e < depth;
+    const f64x2 v = *(gcf64x2*)(base + (ok ? o.b + (unsigned)(e * 8) : last));
+    rb[e] = ok ? v.x : 0.0;
+    rb[e + 1] = ok ? v.y : 0.0;
+  }
+}
+
 template <int N>
 __device__ __forceinline__ void fold(double u, bool first, const double (&x)[N], double (&acc)[N]) {
 #pragma unroll
   for (int e = 0; e < N; ++e) acc[e] = first ? u * x[e] : __builtin_fma(u, x[e], acc[e]);
 }
 
-// contract_rounds for a unit whose depth is a multiple of BK, with 16-byte
-// operand access and an even leaf row count, whose product has NA A terms
-// and NB B terms (both <= kRoundTerms): every
+// contract_rounds for a unit at least BK deep, with 16-byte operand access
+// (so an even leaf depth) and an even leaf row count, whose product has NA A
+// terms and NB B terms (both <= kRoundTerms), over its `nsteps` full slices
+// and, with `tail`, a partial last slice: every
 // load, fold and wait is fixed at compile time, with no branch inside a
 // slice but the one for the last slice, so each fold waits only for its own
 // round's loads (with run-time term counts the waits degrade to vmcnt(0)
 // after every round and the loads of the next round are waited for too).
 template <int BK, int NA, int NB>
 __device__ __forceinline__ void contract_rounds_fixed(const Job& J, const Srcs<kRoundTerms>& S4, int nsteps,
-                                                      const UnitGeo& g, double (*As)[BK][BM],
+                                                      bool tail, const UnitGeo& g, double (*As)[BK][BM],
                                                       double (*Bs)[BN][TileK<BK>::BKP], int lane, int wm, int wn,
                                                       f64x4 (&acc)[FM][FN]) {
   constexpr int NR = NA > NB ? NA : NB;
@@ -931,6 +970,21 @@ __device__ __forceinline__ void contract_rounds_fixed(const Job& J, const Srcs<k
   }
   mfma_slice_hooked<BK>(As[st & 1], Bs[st & 1], lane, wm, wn, acc, [](int) {});
   __syncthreads();
+  if (tail) {
+    // partial last slice, after the full ones (the order of the other
+    // kernels), loaded without overlap
+    const int kb = nsteps * BK;
+#pragma unroll
+    for (int t = 0; t < NR; ++t) {
+      if (t < NA) load_a_tail<BK>(S4.a[t], kb, g.depth, o, ra);
+      if (t < NB) load_b_tail<BK>(S4.b[t], kb, g.depth, o, rb);
+      consume(t);
+    }
+    store_slice<BK>(As[nsteps & 1], Bs[nsteps & 1], g, s.a, s.b);
+    __syncthreads();
+    mfma_slice_hooked<BK>(As[nsteps & 1], Bs[nsteps & 1], lane, wm, wn, acc, [](int) {});
+    __syncthreads();
+  }
 }
 
 template <int BK>
@@ -943,10 +997,12 @@ __device__ __forceinline__ void contract_rounds(const Job& J, const Prod& P, con
   Srcs<kRoundTerms> S4;
   make_srcs<kRoundTerms>(J, P.a0, P.na, P.b0, P.nb, S4);
   const int nr = __builtin_amdgcn_readfirstlane(P.na > P.nb ? P.na : P.nb);
-  if ((J.flags & 3) == 3 && (J.ms & 1) == 0 && g.depth % BK == 0 && nr <= kRoundTerms) {
+  if ((J.flags & 3) == 3 && (J.ms & 1) == 0 && g.depth >= BK && nr <= kRoundTerms) {
+    const int nfull = g.depth / BK;
+    const bool tail = nfull * BK != g.depth;
     switch (P.na * 8 + P.nb) {
 #define FMM_ROUNDS_CASE(a, b) \
-  case (a) * 8 + (b): contract_rounds_fixed<BK, a, b>(J, S4, nsteps, g, As, Bs, lane, wm, wn, acc); return;
+  case (a) * 8 + (b): contract_rounds_fixed<BK, a, b>(J, S4, nfull, tail, g, As, Bs, lane, wm, wn, acc); return;
       FMM_ROUNDS_CASE(1, 1) FMM_ROUNDS_CASE(1, 2) FMM_ROUNDS_CASE(1, 3) FMM_ROUNDS_CASE(1, 4)
       FMM_ROUNDS_CASE(2, 1) FMM_ROUNDS_CASE(2, 2) FMM_ROUNDS_CASE(2, 3) FMM_ROUNDS_CASE(2, 4)
       FMM_ROUNDS_CASE(3, 1) FMM_ROUNDS_CASE(3, 2) FMM_ROUNDS_CASE(3, 3) FMM_ROUNDS_CASE(3, 4)
@@ -1116,7 +1172,10 @@ __global__ void __launch_bounds__(NT) abc_f64_kernel(Launch L) {
   run_units(L, &s_unit, [&](int j, int q, int p) {
     const Job& J = job_at(L, j);
     int tmi, tni;
-    tile_coords<1>(J.tiles_m, J.tiles_n, p, &tmi, &tni);
+    // grouped tile order for the round kernel (measured: 222 L2 16384^3
+    // ABC 134.5 -> 127.3 ms, 333 L1 9600^3 27.35 -> 26.93 ms); row order for
+    // the others (the single-term kernel measured no gain)
+    tile_coords<CH == kRounds ? 4 : 1>(J.tiles_m, J.tiles_n, p, &tmi, &tni);
     UnitGeo g;
     g.row0 = tmi * BM;
     g.col0 = tni * BN;

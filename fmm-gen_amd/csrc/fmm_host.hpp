@@ -787,7 +787,7 @@ inline int launch_jobs_k(JobList JL, int* ws, hipStream_t stream) {
 // a time during the MFMAs, see contract_rounds) runs a launch when every
 // product has at most kRoundTerms terms per operand and every FMM core job
 // takes its compile-time-shaped path (16-byte operand access, even leaf row
-// count, leaf depth a multiple of 32); the chunked kernels serve the rest.
+// count, leaf depth at least 32); the chunked kernels serve the rest.
 // FMM_ABC_ROUNDS=0 selects the chunked kernels always (the two agree bit for
 // bit).  Measured on the 333 L1 9600^3 ABC launch: 32.5 -> 27.3 ms.
 inline bool rounds_enabled() {
@@ -797,7 +797,7 @@ inline bool rounds_enabled() {
 
 inline bool rounds_fit(const JobList& JL) {
   for (const Job& j : JL.jobs)
-    if (j.nprod > 1 && !((j.flags & 3) == 3 && j.ms % 2 == 0 && j.ks % 32 == 0)) return false;
+    if (j.nprod > 1 && !((j.flags & 3) == 3 && j.ms % 2 == 0 && j.ks >= 32)) return false;
   return true;
 }
 

@@ -130,10 +130,11 @@ def env(name, value):
 
 @pytest.mark.parametrize("spec", SPECS, ids=IDS)
 def test_abc_round_kernel_bitwise(torch, fmm, rt, oracle, spec):
-    """The staggered-round ABC kernel (leaf depth a multiple of 32) against
-    the chunked ABC kernels (FMM_ABC_ROUNDS=0): bitwise equal on random
-    inputs, with edge tiles (leaf rows / columns not multiples of 128, whose
-    out-of-range rows and columns are loaded clamped) and k-fringes; exact on
+    """The staggered-round ABC kernel (even leaf depth >= 32) against the
+    chunked ABC kernels (FMM_ABC_ROUNDS=0): bitwise equal on random inputs,
+    with edge tiles (leaf rows / columns not multiples of 128, whose
+    out-of-range rows and columns are loaded clamped), k-fringes and leaf
+    depths that end in a partial 32-deep slice (masked loads); exact on
     integer inputs against numpy and the oracle."""
     alg = fmm.build_algorithm(*spec)
     plan = rt.Plan(alg)
@@ -141,7 +142,9 @@ def test_abc_round_kernel_bitwise(torch, fmm, rt, oracle, spec):
     rng = np.random.default_rng(21)
     for (m, n, k) in [(Mx * 8 * 20, Nx * 4 * 40, Kx * 64),
                       (Mx * 8 * 17 + 6, Nx * 4 * 29 + 1, Kx * 96 + 2),
-                      (Mx * 136, Nx * 132, Kx * 32)]:
+                      (Mx * 136, Nx * 132, Kx * 32),
+                      (Mx * 8 * 9 + 2, Nx * 4 * 11, Kx * 146),
+                      (Mx * 8 * 5, Nx * 4 * 13 + 3, Kx * 50 + 1)]:
         A = rng.uniform(-1, 1, (m, k))
         B = rng.uniform(-1, 1, (k, n))
         C = rng.uniform(-1, 1, (m, n))
