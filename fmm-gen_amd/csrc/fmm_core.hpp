@@ -752,9 +752,18 @@ __device__ __forceinline__ void contract(const Job& J, const Prod& P, const Unit
 // than 4 terms per operand) are loaded and folded at the end of the slice
 // without prefetch.
 constexpr int kRounds = 0;
-constexpr int kRoundTerms = 4;
+constexpr int kRoundTerms = 8;
 
-__device__ __forceinline__ Srcs<1> round_src(const Srcs<kRoundTerms>& S, int t) {
+// k-step after which round t of NR is folded: every second k-step from k-step
+// 1 while the rounds fit (measured best for up to 4 rounds), else one round
+// per k-step.
+template <int KK, int NR>
+constexpr int round_pos(int t) {
+  return 2 * NR <= KK ? 2 * t + 1 : t;
+}
+
+template <int N>
+__device__ __forceinline__ Srcs<1> round_src(const Srcs<N>& S, int t) {
   Srcs<1> R;
   R.na = t < S.na ? 1 : 0;
   R.nb = t < S.nb ? 1 : 0;
@@ -921,12 +930,15 @@ __device__ __forceinline__ void fold(double u, bool first, const double (&x)[N],
 // round's loads (with run-time term counts the waits degrade to vmcnt(0)
 // after every round and the loads of the next round are waited for too).
 template <int BK, int NA, int NB>
-__device__ __forceinline__ void contract_rounds_fixed(const Job& J, const Srcs<kRoundTerms>& S4, int nsteps,
-                                                      bool tail, const UnitGeo& g, double (*As)[BK][BM],
+__device__ __forceinline__ void contract_rounds_fixed(const Job& J, const Prod& P, int nsteps, bool tail,
+                                                      const UnitGeo& g, double (*As)[BK][BM],
                                                       double (*Bs)[BN][TileK<BK>::BKP], int lane, int wm, int wn,
                                                       f64x4 (&acc)[FM][FN]) {
   constexpr int NR = NA > NB ? NA : NB;
-  static_assert(NA >= 1 && NB >= 1 && NR <= kRoundTerms && BK / 4 >= 2 * NR, "round layout");
+  constexpr int KK = BK / 4;
+  static_assert(NA >= 1 && NB >= 1 && NR <= kRoundTerms && NR <= KK, "round layout");
+  Srcs<NR> S4;   // the product's sources, in scalar registers
+  make_srcs<NR>(J, P.a0, P.na, P.b0, P.nb, S4);
   double ra[TileK<BK>::AE], rb[TileK<BK>::BE];
   Slice<BK> s;
   const ThreadOff o = thread_off(J, g);
@@ -953,15 +965,17 @@ __device__ __forceinline__ void contract_rounds_fixed(const Job& J, const Srcs<k
     const int cur = st & 1;
     const int kb1 = (st + 1) * BK;
     mfma_slice_hooked<BK>(As[cur], Bs[cur], lane, wm, wn, acc, [&](int kk) {
-      const int t = kk >> 1;
-      if ((kk & 1) && t < NR) {
-        // keep the machine scheduler from hoisting later rounds' loads
-        // (each hoisted round would hold another raw register set)
-        __builtin_amdgcn_sched_barrier(0);
-        consume(t);
-        __builtin_amdgcn_sched_barrier(0);
-        if (t + 1 < NR) issue(t + 1, kb1);
-        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int t = 0; t < NR; ++t) {
+        if (kk == round_pos<KK, NR>(t)) {
+          // keep the machine scheduler from hoisting later rounds' loads
+          // (each hoisted round would hold another raw register set)
+          __builtin_amdgcn_sched_barrier(0);
+          consume(t);
+          __builtin_amdgcn_sched_barrier(0);
+          if (t + 1 < NR) issue(t + 1, kb1);
+          __builtin_amdgcn_sched_barrier(0);
+        }
       }
     });
     store_slice<BK>(As[cur ^ 1], Bs[cur ^ 1], g, s.a, s.b);
@@ -991,26 +1005,30 @@ template <int BK>
 __device__ __forceinline__ void contract_rounds(const Job& J, const Prod& P, const UnitGeo& g, double (*As)[BK][BM],
                                                 double (*Bs)[BN][TileK<BK>::BKP], int lane, int wm, int wn,
                                                 f64x4 (&acc)[FM][FN]) {
-  static_assert(BK / 4 >= 2 * kRoundTerms, "one round per two k-steps");
+  static_assert(BK / 4 >= kRoundTerms, "one round per k-step");
   const int nsteps = (g.depth + BK - 1) / BK;
   if (!(P.na > 0 && P.nb > 0 && nsteps > 0)) return;
-  Srcs<kRoundTerms> S4;
-  make_srcs<kRoundTerms>(J, P.a0, P.na, P.b0, P.nb, S4);
   const int nr = __builtin_amdgcn_readfirstlane(P.na > P.nb ? P.na : P.nb);
   if ((J.flags & 3) == 3 && (J.ms & 1) == 0 && g.depth >= BK && nr <= kRoundTerms) {
     const int nfull = g.depth / BK;
     const bool tail = nfull * BK != g.depth;
-    switch (P.na * 8 + P.nb) {
+    switch (P.na * 16 + P.nb) {
 #define FMM_ROUNDS_CASE(a, b) \
-  case (a) * 8 + (b): contract_rounds_fixed<BK, a, b>(J, S4, nfull, tail, g, As, Bs, lane, wm, wn, acc); return;
+  case (a) * 16 + (b): contract_rounds_fixed<BK, a, b>(J, P, nfull, tail, g, As, Bs, lane, wm, wn, acc); return;
+      // every term-count pair of <2,2,2>, <2,3,2>, <3,2,3>, <3,3,3> at one
+      // level, <2,2,2> at two and <2,2,2> x <3,2,3> (others: generic path)
       FMM_ROUNDS_CASE(1, 1) FMM_ROUNDS_CASE(1, 2) FMM_ROUNDS_CASE(1, 3) FMM_ROUNDS_CASE(1, 4)
       FMM_ROUNDS_CASE(2, 1) FMM_ROUNDS_CASE(2, 2) FMM_ROUNDS_CASE(2, 3) FMM_ROUNDS_CASE(2, 4)
       FMM_ROUNDS_CASE(3, 1) FMM_ROUNDS_CASE(3, 2) FMM_ROUNDS_CASE(3, 3) FMM_ROUNDS_CASE(3, 4)
       FMM_ROUNDS_CASE(4, 1) FMM_ROUNDS_CASE(4, 2) FMM_ROUNDS_CASE(4, 3) FMM_ROUNDS_CASE(4, 4)
+      FMM_ROUNDS_CASE(2, 8) FMM_ROUNDS_CASE(3, 6) FMM_ROUNDS_CASE(4, 8) FMM_ROUNDS_CASE(6, 3)
+      FMM_ROUNDS_CASE(6, 6) FMM_ROUNDS_CASE(8, 2) FMM_ROUNDS_CASE(8, 4)
 #undef FMM_ROUNDS_CASE
       default: break;
     }
   }
+  Srcs<kRoundTerms> S4;
+  make_srcs<kRoundTerms>(J, P.a0, P.na, P.b0, P.nb, S4);
   Raw<BK, 1> r;
   Slice<BK> s;
   // slice 0, without overlap
@@ -1031,12 +1049,13 @@ __device__ __forceinline__ void contract_rounds(const Job& J, const Prod& P, con
     const bool next = st + 1 < nsteps;
     const int kb1 = (st + 1) * BK;
     mfma_slice_hooked<BK>(As[cur], Bs[cur], lane, wm, wn, acc, [&](int kk) {
-      if ((kk & 1) && next) {
-        const int t = kk >> 1;
-        if (t < kRoundTerms && t < nr) {
-          accumulate<BK, 1>(round_src(S4, t), t == 0, r, s);
-          if (t + 1 < kRoundTerms && t + 1 < nr) load_raw<BK, 1>(J, round_src(S4, t + 1), kb1, g, r);
-        }
+      const int t = kk;
+      if (next && t < kRoundTerms && t < nr) {
+        __builtin_amdgcn_sched_barrier(0);
+        accumulate<BK, 1>(round_src(S4, t), t == 0, r, s);
+        __builtin_amdgcn_sched_barrier(0);
+        if (t + 1 < kRoundTerms && t + 1 < nr) load_raw<BK, 1>(J, round_src(S4, t + 1), kb1, g, r);
+        __builtin_amdgcn_sched_barrier(0);
       }
     });
     if (next) {
@@ -1169,7 +1188,7 @@ __global__ void __launch_bounds__(NT) abc_f64_kernel(Launch L) {
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WAVES_M, wn = wave / WAVES_M;
 
-  run_units(L, &s_unit, [&](int j, int q, int p) {
+  run_units(L, &s_unit, [&](int j, int q, int p) __attribute__((always_inline)) {
     const Job& J = job_at(L, j);
     int tmi, tni;
     // grouped tile order for the round kernel (measured: 222 L2 16384^3

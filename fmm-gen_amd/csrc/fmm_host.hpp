@@ -785,11 +785,12 @@ inline int launch_jobs_k(JobList JL, int* ws, hipStream_t stream) {
 // <3,3,3>, <3,2,3> and <2,2,2>x<3,2,3>.)
 // The staggered-round kernel (32-deep slices, operand sums built one term at
 // a time during the MFMAs, see contract_rounds) runs a launch when every
-// product has at most kRoundTerms terms per operand and every FMM core job
+// product has at most kRoundTerms (8) terms per operand and every FMM core job
 // takes its compile-time-shaped path (16-byte operand access, even leaf row
 // count, leaf depth at least 32); the chunked kernels serve the rest.
 // FMM_ABC_ROUNDS=0 selects the chunked kernels always (the two agree bit for
-// bit).  Measured on the 333 L1 9600^3 ABC launch: 32.5 -> 27.3 ms.
+// bit).  Measured ABC launches at 9600^3: 333 L1 32.5 -> 26.8 ms, 222 x 323
+// 51.5 -> 32.9 ms.
 inline bool rounds_enabled() {
   const char* e = getenv("FMM_ABC_ROUNDS");
   return !(e && e[0] == '0');
