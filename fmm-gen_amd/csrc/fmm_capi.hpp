@@ -28,7 +28,7 @@ inline int run_abc(Plan* P, int dtype, int m, int n, int k, const void* A, int l
                    void* C, int ldc, void* ws, size_t ws_bytes, hipStream_t s, int* err_dst = nullptr) {
   if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
   KGeo G;
-  if (!dtype_geo(dtype, &G)) return kEUnsupported;
+  if (!dtype_geo(dtype, &G, 0, nullptr, P->unit_single)) return kEUnsupported;
   if (m == 0 || n == 0 || k == 0) return kOK;
   Prod* dp;
   Term* dt;
@@ -103,7 +103,7 @@ inline int run_gemm(int dtype, int m, int n, int k, const void* A, int lda, cons
                     hipStream_t s) {
   if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
   KGeo G;
-  if (!dtype_geo(dtype, &G)) return kEUnsupported;
+  if (!dtype_geo(dtype, &G, 0, nullptr, true)) return kEUnsupported;
   if (m == 0 || n == 0 || k == 0) return kOK;
   Prod* ip;
   Term* it;

@@ -18,6 +18,7 @@
 #include "fmm_core.hpp"
 #include "fmm_cs.hpp"
 #include "fmm_lowp.hpp"
+#include "fmm_h16d.hpp"
 
 namespace fmm {
 
@@ -822,16 +823,41 @@ constexpr KGeo lgeo() {
   return KGeo{C::BM, C::BN, C::IN == kInF32 ? 4 : 2, 4, C::AR * (C::IN == kInF32 ? 4 : 2)};
 }
 
+template <class C>
+constexpr KGeo dgeo() {
+  return KGeo{C::BM, C::BN, 2, 4, 16};
+}
+
+// The single-term 16-bit launches run the direct-to-LDS kernel
+// (fmm_h16d.hpp); FMM_H16_DIRECT=0 in the environment selects the
+// register-staged single-term kernel instead (same results bit for bit).
+inline bool h16_direct() {
+  static const bool on = [] {
+    const char* e = getenv("FMM_H16_DIRECT");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 // Geometry of a dtype's kernel for C-tile shape `tile` (0: the default, 1:
 // the transposed 128x256 shape; types without it fall back to the default).
-// *flag receives the job flag that selects the shape.
-inline bool dtype_geo(int dtype, KGeo* g, int tile = 0, int* flag = nullptr) {
+// *flag receives the job flag that selects the shape.  `one`: the jobs run
+// in a single-term launch (launch_jobs_dtype with one = true), whose
+// low-precision kernels have C tiles of their own.
+inline bool dtype_geo(int dtype, KGeo* g, int tile = 0, int* flag = nullptr, bool one = false) {
   int f = 0;
+  const bool o = one && tile == 0;
   switch (dtype) {
     case kF64: *g = kGeoF64; break;
-    case kF32: *g = tile ? lgeo<CfgF32T>() : lgeo<CfgF32>(); f = tile ? kJobTileT : 0; break;
-    case kBF16: *g = tile ? lgeo<CfgBF16T>() : lgeo<CfgBF16>(); f = tile ? kJobTileT : 0; break;
-    case kF16: *g = tile ? lgeo<CfgF16T>() : lgeo<CfgF16>(); f = tile ? kJobTileT : 0; break;
+    case kF32: *g = o ? lgeo<CfgF32One>() : tile ? lgeo<CfgF32T>() : lgeo<CfgF32>(); f = tile ? kJobTileT : 0; break;
+    case kBF16:
+      *g = o ? (h16_direct() ? dgeo<CfgBF16D>() : lgeo<CfgBF16One>()) : tile ? lgeo<CfgBF16T>() : lgeo<CfgBF16>();
+      f = tile ? kJobTileT : 0;
+      break;
+    case kF16:
+      *g = o ? (h16_direct() ? dgeo<CfgF16D>() : lgeo<CfgF16One>()) : tile ? lgeo<CfgF16T>() : lgeo<CfgF16>();
+      f = tile ? kJobTileT : 0;
+      break;
     case kBF16x3: *g = lgeo<CfgBF16x3>(); break;
     default: return false;
   }
@@ -870,9 +896,11 @@ inline int launch_jobs_dtype(int dtype, const JobList& JL, int max_terms, int* w
       return use_one ? launch_lowp<CfgF32One, CfgF32One>(JL, ws, stream)
                      : launch_lowp_shapes<CfgF32, CfgF32T>(JL, ws, stream);
     case kBF16:
+      if (use_one && h16_direct()) return launch_jobs_with(h16d_kernel<CfgBF16D>, CfgBF16D::NT, JL, ws, stream);
       return use_one ? launch_lowp<CfgBF16One, CfgBF16One>(JL, ws, stream)
                      : launch_lowp_shapes<CfgBF16, CfgBF16T>(JL, ws, stream);
     case kF16:
+      if (use_one && h16_direct()) return launch_jobs_with(h16d_kernel<CfgF16D>, CfgF16D::NT, JL, ws, stream);
       return use_one ? launch_lowp<CfgF16One, CfgF16One>(JL, ws, stream)
                      : launch_lowp_shapes<CfgF16, CfgF16T>(JL, ws, stream);
     case kBF16x3: return launch_lowp<CfgBF16x3, CfgBF16x3>(JL, ws, stream);
@@ -1116,7 +1144,7 @@ inline int launch_naive_fused(Plan* P, int dtype, const PackLayout& Lo, int m, i
                               long long lda, const void* B, long long ldb, void* C, long long ldc, void* ws,
                               hipStream_t stream) {
   KGeo G;
-  if (!dtype_geo(dtype, &G)) return kEUnsupported;
+  if (!dtype_geo(dtype, &G, 0, nullptr, true)) return kEUnsupported;
   Prod *dp, *np, *ip;
   Term *dt, *nt, *it;
   int rc = device_tables(P, P->main, &dp, &dt);
@@ -1239,7 +1267,7 @@ inline int launch_pack(Plan* P, int pack, int dtype, const PackLayout& Lo, int m
   if (naive && use_cs(*P, dtype, Lo.ks))
     return launch_naive_cs(P, Lo, m, n, k, A, lda, B, ldb, C, ldc, ws, stream);
   KGeo G;
-  if (!dtype_geo(dtype, &G)) return kEUnsupported;
+  if (!dtype_geo(dtype, &G, 0, nullptr, naive)) return kEUnsupported;
   Prod *dp, *xp, *ip;
   Term *dt, *xt, *it;
   int rc = device_tables(P, P->main, &dp, &dt);

@@ -971,8 +971,15 @@ typedef LC<kInF16, kOpF16, false, 256, 128, 4, 2, FMM_H16_CFG> CfgF16;
 typedef LC<kInF16, kOpF16, false, 128, 256, 2, 4, FMM_H16T_CFG> CfgF16T;
 typedef LC<kInF32, kOpBF16, true, 256, 128, 4, 2, FMM_SPLIT_CFG> CfgBF16x3;
 // single-term (+-1) launches: plain GEMMs and Naive product launches
-typedef LC1<kInF32, kOpF32, 256, 128, 4, 2, FMM_F32_ONE_CFG> CfgF32One;
-typedef LC1<kInBF16, kOpBF16, 256, 128, 4, 2, FMM_H16_ONE_CFG> CfgBF16One;
-typedef LC1<kInF16, kOpF16, 256, 128, 4, 2, FMM_H16_ONE_CFG> CfgF16One;
+// C tile and wave grid (BM, BN, WM, WN) of the single-term kernels
+#ifndef FMM_F32_ONE_SHAPE
+#define FMM_F32_ONE_SHAPE 256, 128, 4, 2
+#endif
+#ifndef FMM_H16_ONE_SHAPE
+#define FMM_H16_ONE_SHAPE 256, 128, 4, 2
+#endif
+typedef LC1<kInF32, kOpF32, FMM_F32_ONE_SHAPE, FMM_F32_ONE_CFG> CfgF32One;
+typedef LC1<kInBF16, kOpBF16, FMM_H16_ONE_SHAPE, FMM_H16_ONE_CFG> CfgBF16One;
+typedef LC1<kInF16, kOpF16, FMM_H16_ONE_SHAPE, FMM_H16_ONE_CFG> CfgF16One;
 
 }  // namespace fmm

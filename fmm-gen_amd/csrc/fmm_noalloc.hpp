@@ -111,7 +111,7 @@ inline int run_abc_noalloc(Plan* P, int dtype, int m, int n, int k, const void* 
                            void* C, int ldc, hipStream_t s) {
   if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
   KGeo G;
-  if (!dtype_geo(dtype, &G)) return kEUnsupported;
+  if (!dtype_geo(dtype, &G, 0, nullptr, P->unit_single)) return kEUnsupported;
   if (m == 0 || n == 0 || k == 0) return kOK;
   // Limits first, on the host: the workspace of this shape and the size of
   // the tables a non-resident plan would copy into the scratch array.
