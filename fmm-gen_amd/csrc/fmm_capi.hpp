@@ -102,6 +102,12 @@ inline size_t workspace_bytes(Plan* P, int pack, int m, int n, int k) {
 inline int run_gemm(int dtype, int m, int n, int k, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
                     hipStream_t s) {
   if (bad_dims(m, n, k, lda, ldb, ldc)) return kEInval;
+  // Split-BF16 with the direct kernel: the identity plan's Naive call splits
+  // A and B into hi / lo BF16 planes (its pre-sums) in a workspace of its own
+  // and runs the product over them; the same products in the same order as
+  // the register-staged split GEMM, so the result is the same bit for bit.
+  if (dtype == kBF16x3 && h16_direct())
+    return run_fmm(identity_plan(), kPackNaive, dtype, m, n, k, A, lda, B, ldb, C, ldc, nullptr, 0, s);
   KGeo G;
   if (!dtype_geo(dtype, &G, 0, nullptr, true)) return kEUnsupported;
   if (m == 0 || n == 0 || k == 0) return kOK;

@@ -66,6 +66,8 @@ struct Job {
                             // kJobStore: C = w*AB instead of C += w*AB
   const Prod* prods;
   const Term* terms;
+  long long alo, blo;       // split-BF16 planes (direct kernel): element offset
+                            // of the lo plane from the hi plane of A / B
 };
 
 constexpr int kInlineJobs = 8;

@@ -41,10 +41,15 @@ typedef __attribute__((address_space(3))) bf16x4v lds_bf16x4v;
 typedef __attribute__((address_space(3))) void lds_void;
 typedef __attribute__((address_space(1))) const void gcvoid;
 
-template <int IN_, int BM_, int BN_, int WM_, int WN_, int BK_, int NST_>
+template <int IN_, int BM_, int BN_, int WM_, int WN_, int BK_, int NST_, bool SPLIT_ = false>
 struct DCfg {
   static constexpr int IN = IN_;
   static constexpr int OP = IN_ == kInBF16 ? kOpBF16 : kOpF16;
+  // split-BF16: A and B arrive as hi and lo BF16 planes (the lo plane at
+  // the job's alo / blo); a product is al*bh + ah*bl + ah*bh per 32-deep
+  // k-step, in the order of the register-staged split kernel
+  static constexpr bool SPLIT = SPLIT_;
+  static constexpr int NPL = SPLIT ? 2 : 1;
   static constexpr int BM = BM_, BN = BN_, BK = BK_;
   static constexpr int WM = WM_, WN = WN_, NST = NST_;
   static constexpr int NT = 64 * WM * WN;
@@ -60,24 +65,34 @@ struct DCfg {
   static constexpr int B_CPI = 1024 / B_ROW;    // B columns per wave load
   static constexpr int A_BYTES = BK * A_ROW;
   static constexpr int B_BYTES = BN * B_ROW;
-  static constexpr int STAGE = A_BYTES + B_BYTES;
+  static constexpr int PLANE = A_BYTES + B_BYTES;    // one operand plane of a stage: A, then B
+  static constexpr int STAGE = NPL * PLANE;
   static constexpr int LDS_BYTES = NST * STAGE;
-  static constexpr int A_LD = A_BYTES / 1024 / NW;   // wave loads per stage and wave
+  static constexpr int A_LD = A_BYTES / 1024 / NW;   // wave loads per stage, plane and wave
   static constexpr int B_LD = B_BYTES / 1024 / NW;
-  static constexpr int LPT = A_LD + B_LD;            // loads per thread per stage
+  static constexpr int LPT = NPL * (A_LD + B_LD);    // loads per thread per stage
   static_assert(IN == kInBF16 || IN == kInF16, "16-bit inputs");
-  static_assert(BK == 32, "B swizzle written for 4 chunks per column");
+  static_assert(!SPLIT || IN == kInBF16, "split planes are BF16");
+  static_assert(BK == 32 || BK == 64, "B swizzle written for 4 or 8 chunks per column");
   static_assert(A_CH >= 16 && A_ROW <= 1024 && 1024 % A_ROW == 0, "A stage rows");
   static_assert(A_LD * NW * 1024 == A_BYTES && B_LD * NW * 1024 == B_BYTES, "whole wave loads");
   static_assert(WTM % 16 == 0 && WTN % 16 == 0, "wave tile");
   static_assert(NST >= 2 && (NST - 2) * LPT <= 63, "vmcnt range");
-  static_assert(LDS_BYTES + 64 <= 160 * 1024, "LDS");
+  static_assert(LDS_BYTES <= 160 * 1024, "LDS");
 };
 
 // Physical 16-byte chunk of logical chunk c of A row k / of logical k group
 // kg of B column j.
+// A: rows k..k+3 of one 16-lane transposing read land 32 bytes apart, rows 8
+// apart in the other half of 256 bytes.  B (NCH chunks per column): 8 (16)
+// consecutive columns at one k group hit distinct 16-byte bank groups of
+// 128 (256) bytes.
 __device__ __forceinline__ int d_swz_a(int k) { return ((k & 3) << 1) ^ (((k >> 3) & 1) << 3); }
-__device__ __forceinline__ int d_swz_b(int j) { return ((j >> 1) ^ (j >> 2)) & 3; }
+template <int NCH>
+__device__ __forceinline__ int d_swz_b(int j) {
+  if constexpr (NCH == 4) return ((j >> 1) ^ (j >> 2)) & 3;
+  else return (j ^ (j >> 3)) & 7;
+}
 
 template <int N>
 __device__ __forceinline__ void d_wait_vm() {
@@ -113,7 +128,7 @@ __device__ __forceinline__ void d_a_chunk(int wave, int lane, int t, int* k, int
 template <class C>
 __device__ __forceinline__ void d_b_chunk(int wave, int lane, int t, int* jj, int* kg) {
   *jj = (wave + C::NW * t) * C::B_CPI + lane / C::B_CH;
-  *kg = (lane % C::B_CH) ^ d_swz_b(*jj);
+  *kg = (lane % C::B_CH) ^ d_swz_b<C::B_CH>(*jj);
 }
 
 // Job fields the contraction uses, held in (scalar) registers.
@@ -165,24 +180,34 @@ __device__ __forceinline__ unsigned lds_u32(const char* p) {
 // Direct loads of stage kb into LDS stage buffer st.
 template <class C>
 __device__ __forceinline__ void d_issue(const typename InT<C::IN>::T* a, const typename InT<C::IN>::T* b,
-                                        long long lda, int kb, const DGeo<C>& g, char* st, int wave) {
-  const void* ab = unip(a + (long long)kb * lda);
-  const void* bb = unip(b + kb);
+                                        long long lda, long long alo, long long blo, int kb, const DGeo<C>& g,
+                                        char* st, int wave) {
   const unsigned l0 = __builtin_amdgcn_readfirstlane(lds_u32(st) + wave * 1024);
 #pragma unroll
-  for (int t = 0; t < C::A_LD; ++t) d_dma16(l0 + t * C::NW * 1024, g.a[t], ab);
+  for (int pl = 0; pl < C::NPL; ++pl) {
+    const void* ab = unip(a + (long long)kb * lda + pl * alo);
+    const void* bb = unip(b + kb + pl * blo);
+    const unsigned lp = l0 + pl * C::PLANE;
 #pragma unroll
-  for (int t = 0; t < C::B_LD; ++t) d_dma16(l0 + C::A_BYTES + t * C::NW * 1024, g.b[t], bb);
+    for (int t = 0; t < C::A_LD; ++t) d_dma16(lp + t * C::NW * 1024, g.a[t], ab);
+#pragma unroll
+    for (int t = 0; t < C::B_LD; ++t) d_dma16(lp + C::A_BYTES + t * C::NW * 1024, g.b[t], bb);
+  }
 }
 
 // Register-staged loads of stage kb (rem valid k, rows / columns past the
 // leaf block and k past rem read as zero) into the same LDS layout.
 // a, b: the A block at row0, the B block at column col0.
 template <class C>
-__device__ __forceinline__ void d_stage_slow(const DDims& J, const typename InT<C::IN>::T* a,
-                                             const typename InT<C::IN>::T* b, int row0, int col0, int kb, int rem,
-                                             char* st, int wave, int lane) {
+__device__ __forceinline__ void d_stage_slow(const DDims& J, const typename InT<C::IN>::T* a0,
+                                             const typename InT<C::IN>::T* b0, long long alo, long long blo,
+                                             int row0, int col0, int kb, int rem, char* st0, int wave, int lane) {
   typedef typename InT<C::IN>::T T;
+#pragma unroll
+  for (int pl = 0; pl < C::NPL; ++pl) {
+  const T* a = a0 + pl * alo;
+  const T* b = b0 + pl * blo;
+  char* st = st0 + pl * C::PLANE;
 #pragma unroll
   for (int t = 0; t < C::A_LD; ++t) {
     int k, c;
@@ -204,6 +229,7 @@ __device__ __forceinline__ void d_stage_slow(const DDims& J, const typename InT<
 #pragma unroll
     for (int e = 0; e < 8; ++e) put_elem<C::IN>(w, e, cok && 8 * kg + e < rem, p + e);
     *reinterpret_cast<u32x4*>(st + C::A_BYTES + (wave + C::NW * t) * 1024 + lane * 16) = u32x4{w[0], w[1], w[2], w[3]};
+  }
   }
 }
 
@@ -231,27 +257,55 @@ __device__ __forceinline__ void d_mfma1(const u32x4& af, const u32x4& bf, f32x4&
 // MFMAs of one stage (KS k-steps of 32); the A operand of block mi+1 is read
 // while the MFMAs of block mi issue.
 template <class C>
-__device__ __forceinline__ void d_mfma(const char* st, const int (&aofs)[C::FM], const int (&bofs)[C::FN],
+__device__ __forceinline__ void d_mfma(const char* st, const int (&aofs)[C::FM], const int (&bofs)[C::KS][C::FN],
                                        f32x4 (&acc)[C::FM][C::FN]) {
   const char* As = st;
   const char* Bs = st + C::A_BYTES;
-  __builtin_amdgcn_s_setprio(1);
+#ifndef FMM_H16D_PRIO
+#define FMM_H16D_PRIO 1
+#endif
+  if (FMM_H16D_PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
   for (int kk = 0; kk < C::KS; ++kk) {
-    u32x4 bf[C::FN];
-#pragma unroll
-    for (int ni = 0; ni < C::FN; ++ni) bf[ni] = *reinterpret_cast<const u32x4*>(Bs + bofs[ni] + kk * 64);
     const char* pa = As + kk * 32 * C::A_ROW;
-    u32x4 af[2];
-    af[0] = d_afrag<C>(pa + aofs[0]);
+    if constexpr (C::SPLIT) {
+      u32x4 bh[C::FN], bl[C::FN];
 #pragma unroll
-    for (int mi = 0; mi < C::FM; ++mi) {
-      if (mi + 1 < C::FM) af[(mi + 1) & 1] = d_afrag<C>(pa + aofs[mi + 1]);
+      for (int ni = 0; ni < C::FN; ++ni) {
+        bh[ni] = *reinterpret_cast<const u32x4*>(Bs + bofs[kk][ni]);
+        bl[ni] = *reinterpret_cast<const u32x4*>(Bs + C::PLANE + bofs[kk][ni]);
+      }
+      u32x4 ah[2], al[2];
+      ah[0] = d_afrag<C>(pa + aofs[0]);
+      al[0] = d_afrag<C>(pa + C::PLANE + aofs[0]);
 #pragma unroll
-      for (int ni = 0; ni < C::FN; ++ni) d_mfma1<C>(af[mi & 1], bf[ni], acc[mi][ni]);
+      for (int mi = 0; mi < C::FM; ++mi) {
+        if (mi + 1 < C::FM) {
+          ah[(mi + 1) & 1] = d_afrag<C>(pa + aofs[mi + 1]);
+          al[(mi + 1) & 1] = d_afrag<C>(pa + C::PLANE + aofs[mi + 1]);
+        }
+#pragma unroll
+        for (int ni = 0; ni < C::FN; ++ni) {
+          d_mfma1<C>(al[mi & 1], bh[ni], acc[mi][ni]);
+          d_mfma1<C>(ah[mi & 1], bl[ni], acc[mi][ni]);
+          d_mfma1<C>(ah[mi & 1], bh[ni], acc[mi][ni]);
+        }
+      }
+    } else {
+      u32x4 bf[C::FN];
+#pragma unroll
+      for (int ni = 0; ni < C::FN; ++ni) bf[ni] = *reinterpret_cast<const u32x4*>(Bs + bofs[kk][ni]);
+      u32x4 af[2];
+      af[0] = d_afrag<C>(pa + aofs[0]);
+#pragma unroll
+      for (int mi = 0; mi < C::FM; ++mi) {
+        if (mi + 1 < C::FM) af[(mi + 1) & 1] = d_afrag<C>(pa + aofs[mi + 1]);
+#pragma unroll
+        for (int ni = 0; ni < C::FN; ++ni) d_mfma1<C>(af[mi & 1], bf[ni], acc[mi][ni]);
+      }
     }
   }
-  __builtin_amdgcn_s_setprio(0);
+  if (FMM_H16D_PRIO) __builtin_amdgcn_s_setprio(0);
 }
 
 // Contraction of one single-term product over `depth`.
@@ -273,13 +327,14 @@ __device__ __forceinline__ void d_contract(const Job& J, const Prod& P, int row0
   // A, and 32-bit lane offsets
   const bool dma = (__builtin_amdgcn_readfirstlane(J.flags) & 3) == 3 && (D.ms & 7) == 0 && D.lda < (1ll << 25) &&
                    D.ldb < (1ll << 23);
+  const long long alo = C::SPLIT ? uni64(J.alo) : 0, blo = C::SPLIT ? uni64(J.blo) : 0;
   const T* at = unip(a + row0);                     // A block at the tile's first row
   const T* bt = unip(b + (long long)col0 * D.ldb);  // B block at the tile's first column
   DGeo<C> g;
   if (dma) d_geo<C>(D, row0, col0, wave, lane, g);
   // fragment offsets within a stage
   const int l16 = lane & 15, lg = lane >> 4;
-  int aofs[C::FM], bofs[C::FN];
+  int aofs[C::FM], bofs[C::KS][C::FN];
   {
     const int r = l16 >> 2, q = l16 & 3;
     const int x = d_swz_a(8 * lg + r);
@@ -289,17 +344,19 @@ __device__ __forceinline__ void d_contract(const Job& J, const Prod& P, int row0
       aofs[mi] = (8 * lg + r) * C::A_ROW + ((c ^ x) << 4) + ((q & 1) << 3);
     }
 #pragma unroll
-    for (int ni = 0; ni < C::FN; ++ni) {
-      const int j = wn * C::WTN + ni * 16 + l16;
-      bofs[ni] = j * C::B_ROW + ((lg ^ d_swz_b(j)) << 4);
-    }
+    for (int kk = 0; kk < C::KS; ++kk)
+#pragma unroll
+      for (int ni = 0; ni < C::FN; ++ni) {
+        const int j = wn * C::WTN + ni * 16 + l16;
+        bofs[kk][ni] = j * C::B_ROW + (((4 * kk + lg) ^ d_swz_b<C::B_CH>(j)) << 4);
+      }
   }
   const int nfull = depth / C::BK;
   const int rem = depth - nfull * C::BK;
   auto stage = [&](int s) { return lds + (s % C::NST) * C::STAGE; };
   auto fill = [&](int s) {
-    if (dma) d_issue<C>(at, bt, D.lda, s * C::BK, g, stage(s), wave);
-    else d_stage_slow<C>(D, at, bt, row0, col0, s * C::BK, C::BK, stage(s), wave, lane);
+    if (dma) d_issue<C>(at, bt, D.lda, alo, blo, s * C::BK, g, stage(s), wave);
+    else d_stage_slow<C>(D, at, bt, alo, blo, row0, col0, s * C::BK, C::BK, stage(s), wave, lane);
   };
 #pragma unroll
   for (int s = 0; s < C::NST - 1; ++s)
@@ -312,7 +369,7 @@ __device__ __forceinline__ void d_contract(const Job& J, const Prod& P, int row0
   }
   if (rem > 0) {
     __syncthreads();
-    d_stage_slow<C>(D, at, bt, row0, col0, nfull * C::BK, rem, stage(nfull), wave, lane);
+    d_stage_slow<C>(D, at, bt, alo, blo, row0, col0, nfull * C::BK, rem, stage(nfull), wave, lane);
     __syncthreads();
     d_mfma<C>(stage(nfull), aofs, bofs, acc);
   }
@@ -359,12 +416,15 @@ __device__ __forceinline__ void d_unit(const Launch& L, const Job& J, int q, int
   if (ordered) pass_turn(ctr, q + 1, L);
 }
 
+// The unit ticket is broadcast through the first word of the last stage
+// buffer: it is read (and a barrier passed) before a unit's first loads, which
+// fill stages 0..NST-2, and the last stage is first written one barrier later.
 template <class C>
 __global__ void __launch_bounds__(C::NT) h16d_kernel(Launch L) {
   __shared__ __align__(16) char lds[C::LDS_BYTES];
-  __shared__ int s_unit;
   const int tid = threadIdx.x;
-  run_units(L, &s_unit, [&](int j, int q, int p) { d_unit<C>(L, job_at(L, j), q, p, lds, tid); });
+  run_units(L, reinterpret_cast<int*>(lds + (C::NST - 1) * C::STAGE),
+            [&](int j, int q, int p) { d_unit<C>(L, job_at(L, j), q, p, lds, tid); });
 }
 
 // C tile, wave grid, stage depth and ring depth of the direct kernels
@@ -373,5 +433,10 @@ __global__ void __launch_bounds__(C::NT) h16d_kernel(Launch L) {
 #endif
 typedef DCfg<kInBF16, FMM_H16D_CFG> CfgBF16D;
 typedef DCfg<kInF16, FMM_H16D_CFG> CfgF16D;
+// split-BF16: two planes per stage (64 KiB), so two stages
+#ifndef FMM_SPLITD_CFG
+#define FMM_SPLITD_CFG 256, 256, 2, 4, 32, 2
+#endif
+typedef DCfg<kInBF16, FMM_SPLITD_CFG, true> CfgSplitD;
 
 }  // namespace fmm

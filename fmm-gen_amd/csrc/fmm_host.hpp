@@ -1123,6 +1123,12 @@ inline int launch_presums(const Plan* P, int dtype, const PackLayout& Lo, const 
     } else if (dtype == kF16) {
       presum_lowp_kernel<kInF16><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
       presum_lowp_kernel<kInF16><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
+    } else if (dtype == kBF16x3 && h16_direct()) {
+      // split planes for the direct kernel: hi at S (T), lo one plane later
+      sa.plane = (long long)Lo.lds * Lo.ks * P->R;
+      sb.plane = (long long)Lo.ldt * Lo.ns * P->R;
+      presum_lowp_kernel<kInF32, true><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
+      presum_lowp_kernel<kInF32, true><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
     } else {
       presum_lowp_kernel<kInF32><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
       presum_lowp_kernel<kInF32><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
@@ -1143,8 +1149,14 @@ inline int launch_presums(const Plan* P, int dtype, const PackLayout& Lo, const 
 inline int launch_naive_fused(Plan* P, int dtype, const PackLayout& Lo, int m, int n, int k, const void* A,
                               long long lda, const void* B, long long ldb, void* C, long long ldc, void* ws,
                               hipStream_t stream) {
-  KGeo G;
-  if (!dtype_geo(dtype, &G, 0, nullptr, true)) return kEUnsupported;
+  // Split-BF16 with the direct kernel: the pre-sums are stored as hi / lo
+  // BF16 planes, which only the product jobs read; the fringe jobs (the
+  // caller's FP32 operands) run in the second launch with the
+  // register-staged split kernel (geometry GF).
+  const bool sd = dtype == kBF16x3 && h16_direct();
+  KGeo GF;
+  if (!dtype_geo(dtype, &GF, 0, nullptr, true)) return kEUnsupported;
+  const KGeo G = sd ? dgeo<CfgSplitD>() : GF;
   Prod *dp, *np, *ip;
   Term *dt, *nt, *it;
   int rc = device_tables(P, P->main, &dp, &dt);
@@ -1156,10 +1168,13 @@ inline int launch_naive_fused(Plan* P, int dtype, const PackLayout& Lo, int m, i
   void* T = w + Lo.t_off;
   rc = launch_presums(P, dtype, Lo, dp, dt, A, lda, B, ldb, S, T, stream);
   if (rc != kOK) return rc;
-  JobList JL;
-  add_gemm_job(JL, G, ip, it, m, n - Lo.nd, k, A, lda, eoff(B, (long long)Lo.nd * ldb, G.sin), ldb,
-               eoff(C, (long long)Lo.nd * ldc, G.sc), ldc);
-  add_gemm_job(JL, G, ip, it, m - Lo.md, Lo.nd, k, eoff(A, Lo.md, G.sin), lda, B, ldb, eoff(C, Lo.md, G.sc), ldc);
+  JobList JL;   // the product launch
+  JobList JF;   // the launch after it: the k-fringe products (and, split, the fringe GEMMs)
+  JobList& JG = sd ? JF : JL;
+  add_gemm_job(JG, GF, ip, it, m, n - Lo.nd, k, A, lda, eoff(B, (long long)Lo.nd * ldb, GF.sin), ldb,
+               eoff(C, (long long)Lo.nd * ldc, GF.sc), ldc);
+  add_gemm_job(JG, GF, ip, it, m - Lo.md, Lo.nd, k, eoff(A, Lo.md, GF.sin), lda, B, ldb, eoff(C, Lo.md, GF.sc),
+               ldc);
   Job j{};
   j.A = S;
   j.B = T;
@@ -1181,6 +1196,10 @@ inline int launch_naive_fused(Plan* P, int dtype, const PackLayout& Lo, int m, i
   j.flags = vec_flags(G, S, Lo.lds, T, Lo.ldt, 0, C, ldc);   // T_r is block (0, r): no row offset
   j.prods = np;
   j.terms = nt;
+  if (sd) {
+    j.alo = (long long)Lo.lds * Lo.ks * P->R;
+    j.blo = (long long)Lo.ldt * Lo.ns * P->R;
+  }
   if (P->R > 1) {
     JL.counters += j.tiles_m * j.tiles_n;
     JL.ordered = true;
@@ -1188,26 +1207,32 @@ inline int launch_naive_fused(Plan* P, int dtype, const PackLayout& Lo, int m, i
   JL.total_units += j.nprod * j.tiles_m * j.tiles_n;
   JL.jobs.push_back(j);
   if (ws_ints(JL) > (size_t)Lo.o_ints) return kEInval;   // the layout reserves 128 x 128 tiles
-  rc = launch_jobs_dtype(dtype, JL, 1, reinterpret_cast<int*>(w + Lo.o_off), stream, true);
-  if (rc != kOK || Lo.kr == 0) return rc;
-  // C(I,J) += A(I, Kx) B(Kx, J) over depth kr: one product per C block, so
-  // the launch needs no ordering.
-  JobList JF;
-  Job f = j;
-  f.A = A;
-  f.B = B;
-  f.lda = lda;
-  f.ldb = ldb;
-  f.kr = Lo.kr;
-  f.nprod = P->Mx * P->Nx;
-  f.ngroups = f.nprod;
-  f.unit_base = 0;
-  f.order_base = -1;
-  f.flags = vec_flags(G, A, lda, B, ldb, Lo.ks, C, ldc);
-  f.prods = dp + P->R;
-  f.terms = dt;
-  JF.total_units = f.nprod * f.tiles_m * f.tiles_n;
-  JF.jobs.push_back(f);
+  int* wo = reinterpret_cast<int*>(w + Lo.o_off);
+  rc = sd ? launch_jobs_with(h16d_kernel<CfgSplitD>, CfgSplitD::NT, JL, wo, stream)
+          : launch_jobs_dtype(dtype, JL, 1, wo, stream, true);
+  if (rc != kOK) return rc;
+  if (Lo.kr > 0) {
+    // C(I,J) += A(I, Kx) B(Kx, J) over depth kr: one product per C block, so
+    // the launch needs no ordering.
+    Job f = j;
+    f.A = A;
+    f.B = B;
+    f.lda = lda;
+    f.ldb = ldb;
+    f.kr = Lo.kr;
+    f.tiles_m = tiles(Lo.ms, GF.bm);
+    f.tiles_n = tiles(Lo.ns, GF.bn);
+    f.nprod = P->Mx * P->Nx;
+    f.ngroups = f.nprod;
+    f.unit_base = JF.total_units;
+    f.order_base = -1;
+    f.flags = vec_flags(GF, A, lda, B, ldb, Lo.ks, C, ldc);
+    f.prods = dp + P->R;
+    f.terms = dt;
+    f.alo = f.blo = 0;
+    JF.total_units += f.nprod * f.tiles_m * f.tiles_n;
+    JF.jobs.push_back(f);
+  }
   return launch_jobs_dtype(dtype, JF, 1, nullptr, stream, true);
 }
 

@@ -820,6 +820,7 @@ struct SumJobL {
   int pad0, pad1;
   const Prod* prods;
   const Term* terms;
+  long long plane;          // split output: element offset of the lo plane
 };
 
 template <int IN>
@@ -835,9 +836,13 @@ __device__ __forceinline__ void store_in(typename InT<IN>::T* p, float v) {
   }
 }
 
-template <int IN>
+// SPLIT (FP32 inputs): the sum is stored as two BF16 planes, hi = bf16(s)
+// at Y and lo = bf16(s - hi) at Y + plane -- the split the split-BF16
+// kernels apply to an FP32 operand in registers.
+template <int IN, bool SPLIT = false>
 __global__ void __launch_bounds__(kEwThreads) presum_lowp_kernel(SumJobL S) {
   typedef typename InT<IN>::T T;
+  static_assert(!SPLIT || IN == kInF32, "split planes from FP32 sums");
   const int r = (int)blockIdx.y;
   int r0, c0;
   ew_tile(S.rows, S.cols, &r0, &c0);
@@ -860,9 +865,23 @@ __global__ void __launch_bounds__(kEwThreads) presum_lowp_kernel(SumJobL S) {
       s0 = t == 0 ? u * x0 : __builtin_fmaf(u, x0, s0);
       s1 = t == 0 ? u * x1 : __builtin_fmaf(u, x1, s1);
     }
-    T* yc = y + (long long)c * S.ldy;
-    if (i < S.rows) store_in<IN>(yc + i, s0);
-    if (i + 1 < S.rows) store_in<IN>(yc + i + 1, s1);
+    if constexpr (SPLIT) {
+      const long long off = (long long)(r % S.ny) * S.rows + (long long)(r / S.ny) * S.cols * S.ldy +
+                            (long long)c * S.ldy;
+      __attribute__((address_space(1))) u16* hc = (__attribute__((address_space(1))) u16*)S.Y + off;
+      auto put = [&](int e, float v) {
+        const __bf16 h = (__bf16)v;
+        const __bf16 l = (__bf16)(v - (float)h);
+        hc[e] = __builtin_bit_cast(u16, h);
+        hc[e + S.plane] = __builtin_bit_cast(u16, l);
+      };
+      if (i < S.rows) put(i, s0);
+      if (i + 1 < S.rows) put(i + 1, s1);
+    } else {
+      T* yc = y + (long long)c * S.ldy;
+      if (i < S.rows) store_in<IN>(yc + i, s0);
+      if (i + 1 < S.rows) store_in<IN>(yc + i + 1, s1);
+    }
   }
 }
 
