@@ -20,6 +20,8 @@ headline in the same line:
              split-BF16 products (FP32 accumulation), and FP64 refinement
              (fmm_refine); effective TFLOP/s and relative error against the
              FP64 product of the original inputs (sampled entries);
+  lowp_margin  16384^3, BF16 and split-BF16: plain GEMM against <2,2,2> L1
+             and L2 in the Naive pack type (median of 5 runs, errors);
   batched    1024 FP32 problems, m, n, k log-uniform in [192, 4096], one
              persistent launch with per-problem choices of the autotuner
              (calibrated on the device first) against all-GEMM batches and
@@ -68,8 +70,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-gemm", action="store_true", help="skip the plain-GEMM comparison")
-    ap.add_argument("--sections", default="headline,config2,margin,precision,batched,serving",
-                    help="comma list of headline, config2, margin, precision, batched, serving")
+    ap.add_argument("--sections", default="headline,config2,margin,precision,lowp_margin,batched,serving",
+                    help="comma list of headline, config2, margin, precision, lowp_margin, batched, serving")
     ap.add_argument("--runs", type=int, default=10, help="timed runs per variant in config2 / margin")
     ap.add_argument("--packs", default="abc,ab,naive", help="headline: pack types to measure")
     ap.add_argument("--gemm-only", action="store_true",
@@ -339,6 +341,61 @@ def section_precision(F, rt, seed, only=()):
             "rows": rows, "best_vs_gemm": best, "equal_error": equal}
 
 
+def section_lowp_margin(F, rt, seed, N=16384, runs=5):
+    """The 16-bit types at 16384^3 (beside the 8192^3 precision sweep): plain
+    GEMM against the <2,2,2> FMMs at one and two levels, Naive pack type (the
+    fastest low-precision pack), BF16 and split-BF16; median of `runs`
+    event-timed calls and the sampled relative error against the FP64
+    product."""
+    import numpy as np
+    import torch
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    A64 = rt.colmajor(N, N)
+    B64 = rt.colmajor(N, N)
+    A64.uniform_(-1.0, 1.0, generator=gen)
+    B64.uniform_(-1.0, 1.0, generator=gen)
+    Z64 = rt.colmajor(N, N).zero_()
+    plans = {"gemm": None, "222L1": rt.Plan(F.build_algorithm(["222"], [1])),
+             "222L2": rt.Plan(F.build_algorithm(["222"], [2]))}
+    fl = 2.0 * N ** 3
+    C = rt.colmajor(N, N, dtype=torch.float32)
+    rows = []
+    for dt in ("bf16", "bf16x3"):
+        ti = torch.bfloat16 if dt == "bf16" else torch.float32
+        A = rt.as_colmajor(A64.to(ti))
+        B = rt.as_colmajor(B64.to(ti))
+        split = dt == "bf16x3"
+        for a, plan in plans.items():
+            pack = "gemm" if plan is None else "naive"
+            ws = None if plan is None else torch.empty(plan.workspace_bytes(N, N, N, pack) + 256,
+                                                       dtype=torch.uint8, device="cuda")
+
+            def call():
+                if plan is None:
+                    rt.gemm(A, B, C, split=split)
+                else:
+                    plan.run(A, B, C, pack=pack, ws=ws, split=split)
+            C.zero_()
+            call()
+            torch.cuda.synchronize()
+            err = rel_sampled(A64, B64, Z64, C)
+            ts = [ev_ms(call, 1) for _ in range(runs)]
+            ms = float(np.median(ts))
+            rows.append({"dtype": dt, "alg": a, "pack": pack, "ms_median": round(ms, 3), "ms_best": round(min(ts), 3),
+                         "eff_tflops": round(fl / ms / 1e9, 1), "rel_err": float(f"{err:.3e}")})
+            del ws
+        del A, B
+    best = {}
+    for dt in ("bf16", "bf16x3"):
+        g = next(r for r in rows if r["dtype"] == dt and r["alg"] == "gemm")
+        f = max((r for r in rows if r["dtype"] == dt and r["alg"] != "gemm"), key=lambda r: r["eff_tflops"])
+        best[dt] = {"best_fmm": f"{f['alg']}/{f['pack']}", "fmm_tflops": f["eff_tflops"], "gemm_tflops": g["eff_tflops"],
+                    "speedup": round(f["eff_tflops"] / g["eff_tflops"], 3)}
+    return {"shape": [N, N, N], "runs": runs, "inputs": "FP64 U[-1,1) cast per dtype, C = 0",
+            "err_metric": "relative Frobenius error vs the FP64 product of the FP64 inputs, 512 sampled entries",
+            "rows": rows, "best_vs_gemm": best}
+
+
 def section_batched(F, rt, seed):
     """Config 'batched': 1024 FP32 problems in one persistent launch."""
     import numpy as np
@@ -457,6 +514,9 @@ def main():
     if "precision" in sections:
         extra["precision_sweep"] = section_precision(F, rt, args.seed,
                                                      tuple(x for x in args.prec_filter.split(",") if x))
+        torch.cuda.empty_cache()
+    if "lowp_margin" in sections:
+        extra["lowp_margin"] = section_lowp_margin(F, rt, args.seed)
         torch.cuda.empty_cache()
     if "batched" in sections:
         extra["batched"] = section_batched(F, rt, args.seed)

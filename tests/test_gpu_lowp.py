@@ -194,3 +194,32 @@ def test_lowp_packs_equal_abc(torch, fmm, rt, oracle, spec, dtype, pack):
         plan.run(dA, dB, dC, pack=pack, split=split)
         torch.cuda.synchronize()
         assert np.array_equal(dC.cpu().numpy(), Ci + Ai @ Bi), (spec, dtype, pack, "int", m, n, k)
+
+
+def sign_algorithm(fmm):
+    """<1,2,1> with two one-term products whose operands carry -1:
+    M1 = (-A0) B0, M2 = A1 (-B1), C = -M1 - M2 = A0 B0 + A1 B1."""
+    from fractions import Fraction as Fr
+    return fmm.Algorithm(name="sign121", levels=[(1, 2, 1)], U=[[Fr(-1), Fr(0)], [Fr(0), Fr(1)]],
+                         V=[[Fr(1), Fr(0)], [Fr(0), Fr(-1)]], W=[[Fr(-1), Fr(-1)]])
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "f16", "bf16x3"])
+def test_lowp_single_term_direct_paths(torch, fmm, rt, dtype):
+    """The single-term 16-bit launches (direct-to-LDS kernel): plain GEMMs with
+    aligned and odd leading dimensions (odd ones take the register-staged
+    loads), edge tiles and partial last k stages; and products whose single
+    operand terms carry -1 (the sign is applied to the accumulators), two of
+    them on every C tile in product order."""
+    rng = np.random.default_rng(17)
+    for (m, n, k) in [(264, 136, 100), (520, 300, 96), (40, 520, 33)]:
+        A, B, C = int_mats(rng, m, n, k)
+        for pad in (0, 1, 3, 8):
+            got = run(torch, rt, dtype, A, B, C, ldpad=pad)
+            assert np.array_equal(got, C + A @ B), (dtype, m, n, k, pad)
+    plan = rt.Plan(sign_algorithm(fmm))
+    for (m, n, k) in [(512, 512, 256), (264, 300, 130)]:
+        A, B, C = int_mats(rng, m, n, k)
+        for pad in (0, 3):
+            got = run(torch, rt, dtype, A, B, C, plan=plan, ldpad=pad)
+            assert np.array_equal(got, C + A @ B), (dtype, "sign", m, n, k, pad)
