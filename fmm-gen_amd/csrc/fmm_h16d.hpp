@@ -77,7 +77,14 @@ struct DCfg {
   static_assert(A_CH >= 16 && A_ROW <= 1024 && 1024 % A_ROW == 0, "A stage rows");
   static_assert(A_LD * NW * 1024 == A_BYTES && B_LD * NW * 1024 == B_BYTES, "whole wave loads");
   static_assert(WTM % 16 == 0 && WTN % 16 == 0, "wave tile");
-  static_assert(NST >= 2 && (NST - 2) * LPT <= 63, "vmcnt range");
+  static_assert(NST >= 2 && (NST - 1) * LPT <= 63, "vmcnt range");
+  // A blocks whose MFMAs run after the barrier that ends a stage (their A
+  // operands are read before it), hiding the barrier and the next stage's
+  // first operand reads behind them
+#ifndef FMM_H16D_TAIL
+#define FMM_H16D_TAIL 2
+#endif
+  static constexpr int TAILM = FMM_H16D_TAIL < FM ? FMM_H16D_TAIL : FM - 1;
   static_assert(LDS_BYTES <= 160 * 1024, "LDS");
 };
 
@@ -100,21 +107,17 @@ __device__ __forceinline__ void d_wait_vm() {
 }
 
 // Wait until at most n stages of this thread's loads are outstanding.
-template <class C>
+template <class C, int N = C::NST - 1>
 __device__ __forceinline__ void d_wait_stages(int n) {
-  if constexpr (C::NST > 3) {
-    if (n >= 2) {
-      d_wait_vm<2 * C::LPT>();
+  if constexpr (N > 0) {
+    if (n >= N) {
+      d_wait_vm<N * C::LPT>();
       return;
     }
+    d_wait_stages<C, N - 1>(n);
+  } else {
+    d_wait_vm<0>();
   }
-  if constexpr (C::NST > 2) {
-    if (n >= 1) {
-      d_wait_vm<C::LPT>();
-      return;
-    }
-  }
-  d_wait_vm<0>();
 }
 
 // Which chunk of a stage a lane moves, for wave load t of A (B): the A row
@@ -254,55 +257,76 @@ __device__ __forceinline__ void d_mfma1(const u32x4& af, const u32x4& bf, f32x4&
                                                  0, 0);
 }
 
-// MFMAs of one stage (KS k-steps of 32); the A operand of block mi+1 is read
-// while the MFMAs of block mi issue.
+// B operands of one k-step (hi and lo planes when split).
 template <class C>
-__device__ __forceinline__ void d_mfma(const char* st, const int (&aofs)[C::FM], const int (&bofs)[C::KS][C::FN],
-                                       f32x4 (&acc)[C::FM][C::FN]) {
-  const char* As = st;
-  const char* Bs = st + C::A_BYTES;
+struct DFrag {
+  u32x4 b[C::NPL][C::FN];
+};
+
+template <class C>
+__device__ __forceinline__ void d_bfrags(const char* st, const int (&bofs)[C::KS][C::FN], int kk, DFrag<C>& f) {
+#pragma unroll
+  for (int pl = 0; pl < C::NPL; ++pl)
+#pragma unroll
+    for (int ni = 0; ni < C::FN; ++ni)
+      f.b[pl][ni] = *reinterpret_cast<const u32x4*>(st + pl * C::PLANE + C::A_BYTES + bofs[kk][ni]);
+}
+
+template <class C>
+__device__ __forceinline__ void d_afrags(const char* st, const int (&aofs)[C::FM], int kk, int mi,
+                                         u32x4 (&a)[C::NPL]) {
+#pragma unroll
+  for (int pl = 0; pl < C::NPL; ++pl) a[pl] = d_afrag<C>(st + pl * C::PLANE + kk * 32 * C::A_ROW + aofs[mi]);
+}
+
+// The MFMAs of one row of blocks (split: al*bh, ah*bl, ah*bh, the order of
+// the register-staged split kernel).
+template <class C>
+__device__ __forceinline__ void d_mma(const u32x4 (&a)[C::NPL], const DFrag<C>& f, f32x4 (&acc)[C::FN]) {
+#pragma unroll
+  for (int ni = 0; ni < C::FN; ++ni) {
+    if constexpr (C::SPLIT) {
+      d_mfma1<C>(a[1], f.b[0][ni], acc[ni]);
+      d_mfma1<C>(a[0], f.b[1][ni], acc[ni]);
+    }
+    d_mfma1<C>(a[0], f.b[0][ni], acc[ni]);
+  }
+}
+
+// MFMAs of one stage (KS k-steps of 32); the A operands of block mi+1 are
+// read while the MFMAs of block mi issue.  `hook` runs after every LDS read
+// of the stage has been issued and before the MFMAs of the last TAILM blocks
+// of the last k-step.
+template <class C, class Hook>
+__device__ __forceinline__ void d_stage(const char* st, const int (&aofs)[C::FM], const int (&bofs)[C::KS][C::FN],
+                                        f32x4 (&acc)[C::FM][C::FN], Hook&& hook) {
 #ifndef FMM_H16D_PRIO
 #define FMM_H16D_PRIO 1
 #endif
   if (FMM_H16D_PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
   for (int kk = 0; kk < C::KS; ++kk) {
-    const char* pa = As + kk * 32 * C::A_ROW;
-    if constexpr (C::SPLIT) {
-      u32x4 bh[C::FN], bl[C::FN];
+    constexpr int T0 = C::FM - C::TAILM;
+    const bool last = kk == C::KS - 1;
+    DFrag<C> f;
+    d_bfrags<C>(st, bofs, kk, f);
+    u32x4 a[2][C::NPL];
+    d_afrags<C>(st, aofs, kk, 0, a[0]);
 #pragma unroll
-      for (int ni = 0; ni < C::FN; ++ni) {
-        bh[ni] = *reinterpret_cast<const u32x4*>(Bs + bofs[kk][ni]);
-        bl[ni] = *reinterpret_cast<const u32x4*>(Bs + C::PLANE + bofs[kk][ni]);
-      }
-      u32x4 ah[2], al[2];
-      ah[0] = d_afrag<C>(pa + aofs[0]);
-      al[0] = d_afrag<C>(pa + C::PLANE + aofs[0]);
+    for (int mi = 0; mi < C::FM; ++mi) {
+      if (last && mi == T0) break;
+      if (mi + 1 < C::FM && !(last && mi + 1 == T0)) d_afrags<C>(st, aofs, kk, mi + 1, a[(mi + 1) & 1]);
+      d_mma<C>(a[mi & 1], f, acc[mi]);
+    }
+    if (last) {
+      u32x4 t[C::TAILM][C::NPL];
 #pragma unroll
-      for (int mi = 0; mi < C::FM; ++mi) {
-        if (mi + 1 < C::FM) {
-          ah[(mi + 1) & 1] = d_afrag<C>(pa + aofs[mi + 1]);
-          al[(mi + 1) & 1] = d_afrag<C>(pa + C::PLANE + aofs[mi + 1]);
-        }
+      for (int i = 0; i < C::TAILM; ++i) d_afrags<C>(st, aofs, kk, T0 + i, t[i]);
+      if (FMM_H16D_PRIO) __builtin_amdgcn_s_setprio(0);
+      hook();
+      if (FMM_H16D_PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
-        for (int ni = 0; ni < C::FN; ++ni) {
-          d_mfma1<C>(al[mi & 1], bh[ni], acc[mi][ni]);
-          d_mfma1<C>(ah[mi & 1], bl[ni], acc[mi][ni]);
-          d_mfma1<C>(ah[mi & 1], bh[ni], acc[mi][ni]);
-        }
-      }
-    } else {
-      u32x4 bf[C::FN];
-#pragma unroll
-      for (int ni = 0; ni < C::FN; ++ni) bf[ni] = *reinterpret_cast<const u32x4*>(Bs + bofs[kk][ni]);
-      u32x4 af[2];
-      af[0] = d_afrag<C>(pa + aofs[0]);
-#pragma unroll
-      for (int mi = 0; mi < C::FM; ++mi) {
-        if (mi + 1 < C::FM) af[(mi + 1) & 1] = d_afrag<C>(pa + aofs[mi + 1]);
-#pragma unroll
-        for (int ni = 0; ni < C::FN; ++ni) d_mfma1<C>(af[mi & 1], bf[ni], acc[mi][ni]);
-      }
+      for (int i = 0; i < C::TAILM; ++i) d_mma<C>(t[i], f, acc[T0 + i]);
     }
   }
   if (FMM_H16D_PRIO) __builtin_amdgcn_s_setprio(0);
@@ -358,20 +382,30 @@ __device__ __forceinline__ void d_contract(const Job& J, const Prod& P, int row0
     if (dma) d_issue<C>(at, bt, D.lda, alo, blo, s * C::BK, g, stage(s), wave);
     else d_stage_slow<C>(D, at, bt, alo, blo, row0, col0, s * C::BK, C::BK, stage(s), wave, lane);
   };
+  // Every stage buffer is filled up front; the barrier that publishes stage
+  // s+1 sits inside stage s (after all of stage s's LDS reads, before its
+  // last MFMAs) and frees stage s's buffer for stage s + NST.
 #pragma unroll
-  for (int s = 0; s < C::NST - 1; ++s)
+  for (int s = 0; s < C::NST; ++s)
     if (s < nfull) fill(s);
-  for (int s = 0; s < nfull; ++s) {
-    d_wait_stages<C>(min(C::NST - 2, nfull - 1 - s));
+  if (nfull > 0) {
+    d_wait_stages<C>(min(C::NST - 1, nfull - 1));
     __syncthreads();
-    if (s + C::NST - 1 < nfull) fill(s + C::NST - 1);
-    d_mfma<C>(stage(s), aofs, bofs, acc);
+  }
+  for (int s = 0; s < nfull; ++s) {
+    d_stage<C>(stage(s), aofs, bofs, acc, [&]() {
+      if (s + 1 < nfull) {
+        d_wait_stages<C>(min(C::NST - 2, nfull - 2 - s));
+        __syncthreads();
+        if (s + C::NST < nfull) fill(s + C::NST);
+      }
+    });
   }
   if (rem > 0) {
     __syncthreads();
     d_stage_slow<C>(D, at, bt, alo, blo, row0, col0, nfull * C::BK, rem, stage(nfull), wave, lane);
     __syncthreads();
-    d_mfma<C>(stage(nfull), aofs, bofs, acc);
+    d_stage<C>(stage(nfull), aofs, bofs, acc, []() {});
   }
   __syncthreads();   // the next unit's first loads overwrite stages read here
   if (sgn < 0.0) {
@@ -427,9 +461,15 @@ __global__ void __launch_bounds__(C::NT) h16d_kernel(Launch L) {
             [&](int j, int q, int p) { d_unit<C>(L, job_at(L, j), q, p, lds, tid); });
 }
 
-// C tile, wave grid, stage depth and ring depth of the direct kernels
+// C tile, wave grid, stage depth and ring depth of the direct kernels.
+// Measured (BF16 GEMM, TFLOP/s at 8192^3 / 9600^3 / 16384^3): 32-deep
+// stages in a ring of 4: 1000-1090 / 1030-1070 / 1050-1075; 64-deep stages,
+// ring of 2: 1070-1095 / 1035-1040 / 1135-1150 (the default); 3 stages or
+// 64 x 128 wave tiles or a 256 x 128 C tile: 2-10% slower.  Moving the
+// stage barrier inside the stage (TAILM) is neutral here and +7% for the
+// split kernel.
 #ifndef FMM_H16D_CFG
-#define FMM_H16D_CFG 256, 256, 2, 4, 32, 4
+#define FMM_H16D_CFG 256, 256, 2, 4, 64, 2
 #endif
 typedef DCfg<kInBF16, FMM_H16D_CFG> CfgBF16D;
 typedef DCfg<kInF16, FMM_H16D_CFG> CfgF16D;
