@@ -19,6 +19,7 @@
 #include "fmm_cs.hpp"
 #include "fmm_lowp.hpp"
 #include "fmm_h16d.hpp"
+#include "fmm_f64d.hpp"
 
 namespace fmm {
 
@@ -803,7 +804,19 @@ inline bool rounds_fit(const JobList& JL) {
   return true;
 }
 
+// FP64 launches of single-term products run the direct-to-LDS kernel
+// (fmm_f64d.hpp); FMM_F64_DIRECT=0 in the environment selects the
+// register-staged single-term kernel (same results bit for bit).
+inline bool f64_direct() {
+  static const bool on = [] {
+    const char* e = getenv("FMM_F64_DIRECT");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 inline int launch_jobs(const JobList& JL, int max_terms, int* ws, hipStream_t stream) {
+  if (max_terms <= 1 && f64_direct()) return launch_jobs_with(f64d_kernel<CfgF64D>, NT, JL, ws, stream);
   if (max_terms <= 1) return launch_jobs_k<32, 1>(JL, ws, stream);
   if (max_terms <= kRoundTerms && rounds_enabled() && rounds_fit(JL))
     return launch_jobs_k<32, kRounds>(JL, ws, stream);
