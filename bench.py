@@ -628,7 +628,8 @@ def main():
                 "kernel_ms_mean": round(sum(gper) / len(gper), 4), "max_norm_err": round(err_gemm, 3)}
 
     # roofline of the dominant kernel of the reported FMM call -- its product
-    # launch (abc_f64_kernel), timed alone with HIP events around the launch
+    # launch (f64d_kernel for Naive, abc_f64_kernel for ABC / AB), timed alone
+    # with HIP events around the launch
     # (fmm_timing) over K calls: executed MFMA flops per launch over the mean
     # launch duration.  The same for the ABC pack type (the hot path).
     p = F.partition(m, n, k, alg.dims)
@@ -656,7 +657,9 @@ def main():
         b_min = 8 * (nnzU * p.ms * p.ks + nnzV * p.ks * p.ns) + 8 * 2 * nnzW * p.ms * p.ns
     roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_F64_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_F64_TFLOPS, 4), "traffic": load_traffic(args.workload, best),
-            "kernel": f"abc_f64_kernel, the product launch of the {best} pack type", "flops_per_launch": f_mfma,
+            "kernel": ("f64d_kernel (single-term, direct-to-LDS)" if best == "naive" else
+                       "abc_f64_kernel (operand sums)") + f", the product launch of the {best} pack type",
+            "flops_per_launch": f_mfma,
             "algorithmic_bytes_per_launch": b_min, "kernel_ms_mean": round(kern_ms, 4),
             "abc_kernel": None if abc_ms is None else {
                 "achieved": round(f_mfma / (abc_ms * 1e-3) / 1e12, 2),
