@@ -1330,6 +1330,82 @@ __global__ void __launch_bounds__(kEwThreads) presum_kernel(SumJob S) {
   }
 }
 
+// The Naive pre-sums of one side for every product in one pass: a
+// workgroup stages one column run (kEwRows rows) of every source block of
+// the operand (nbr x nbc blocks) into LDS once -- its waves split the blocks
+// -- and then writes that run of every product's sum, the waves splitting
+// the products.  Each source element is read from HBM once instead of once
+// per product term that names it (333: 9 block reads per element instead of
+// 49; <2,2,2> x <3,2,3>: 24 instead of 384).  Same arithmetic and term order
+// as presum_kernel, so the sums are equal bit for bit.  Dynamic LDS: 1 KiB
+// per source block.
+constexpr int kSaWaves = 4;
+constexpr int kSaMaxBlocks = 128;
+struct SumAllJob {
+  const double* X;
+  double* Y;
+  long long ldx, ldy;
+  int rows, cols;      // leaf block extent
+  int ny;              // blocks per block column of Y
+  int side;            // 0: the A terms of each product, 1: the B terms
+  int vec;             // 1: 16-byte accesses allowed (even rows/ld, aligned)
+  int skip1;           // 1: leave out products with a single term on this side
+  int nbr, nbc;        // source blocks: (bi, bj) with bi < nbr, bj < nbc
+  int nprod;           // products
+  int pad;
+  const Prod* prods;
+  const Term* terms;
+};
+
+__global__ void __launch_bounds__(64 * kSaWaves) presum_all_kernel(SumAllJob S) {
+  extern __shared__ f64x2 sa_lds[];
+  const int tr = (S.rows + kEwRows - 1) / kEwRows;
+  const int t = (int)blockIdx.x;
+  const int r0 = (t % tr) * kEwRows;
+  const int c = t / tr;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = r0 + 2 * lane;
+  const bool v2 = S.vec && i + 1 < S.rows;
+  const int nb = S.nbr * S.nbc;
+  for (int b = wave; b < nb; b += kSaWaves) {
+    const int bi = b / S.nbc, bj = b - bi * S.nbc;
+    gcdouble* x = (gcdouble*)S.X + (long long)bi * S.rows + (long long)bj * S.cols * S.ldx + (long long)c * S.ldx;
+    f64x2 v = f64x2{0.0, 0.0};
+    if (v2) {
+      v = *(gcf64x2*)(x + i);
+    } else {
+      if (i < S.rows) v.x = x[i];
+      if (i + 1 < S.rows) v.y = x[i + 1];
+    }
+    sa_lds[b * 64 + lane] = v;
+  }
+  __syncthreads();
+  for (int r = wave; r < S.nprod; r += kSaWaves) {
+    const Prod* pp = S.prods + r;
+    const int t0 = S.side ? pp->b0 : pp->a0;
+    const int nt = S.side ? pp->nb : pp->na;
+    if (S.skip1 && nt == 1) continue;
+    double s0 = 0.0, s1 = 0.0;
+    for (int u = 0; u < nt; ++u) {
+      const Term tm = S.terms[t0 + u];
+      const f64x2 x = sa_lds[(tm.bi * S.nbc + tm.bj) * 64 + lane];
+      s0 = u == 0 ? tm.coef * x.x : __builtin_fma(tm.coef, x.x, s0);
+      s1 = u == 0 ? tm.coef * x.y : __builtin_fma(tm.coef, x.y, s1);
+    }
+    gdouble* yc = gptr(S.Y) + (long long)(r % S.ny) * S.rows + (long long)(r / S.ny) * S.cols * S.ldy +
+                  (long long)c * S.ldy;
+    if (v2) {
+      f64x2 v;
+      v.x = s0;
+      v.y = s1;
+      *(__attribute__((address_space(1))) f64x2*)(yc + i) = v;
+    } else {
+      if (i < S.rows) yc[i] = s0;
+      if (i + 1 < S.rows) yc[i + 1] = s1;
+    }
+  }
+}
+
 // M_Add of the AB and Naive pack types (meta/naive_gen_mix.py:244-361,
 // meta/ab_gen_mix.py:341-372): for C block b = I*Nx + J (grid.y),
 // C_b = fma(w, M_r, C_b) over the list cptr[b]..cptr[b+1] in product order --

@@ -1078,6 +1078,17 @@ inline bool pack_layout(const Plan& P, int pack, int m, int n, int k, PackLayout
 
 inline int ew_grid(int rows, int cols) { return tiles(rows, kEwRows) * tiles(cols, kEwCols); }
 
+// FMM_PRESUM_ALL=0 in the environment runs the FP64 pre-sums one product
+// per grid row (presum_kernel) instead of one pass per operand
+// (presum_all_kernel); the sums are the same bit for bit.
+inline bool presum_all_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("FMM_PRESUM_ALL");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 // Naive pre-sums S_r = sum_t u_t A_t and T_r = sum_t v_t B_t into the
 // workspace blocks (0, r) of S and T, with the ABC kernel's packing arithmetic.
 inline int launch_presums(const Plan* P, int dtype, const PackLayout& Lo, const Prod* dp, const Term* dt,
@@ -1098,8 +1109,6 @@ inline int launch_presums(const Plan* P, int dtype, const PackLayout& Lo, const 
     sa.skip1 = skip_single ? 1 : 0;
     sa.prods = dp;
     sa.terms = dt;
-    presum_kernel<<<ga, dim3(kEwThreads), 0, stream>>>(sa);
-    if (hipGetLastError() != hipSuccess) return kEHip;
     SumJob sb = sa;
     sb.X = static_cast<const double*>(B);
     sb.Y = static_cast<double*>(T);
@@ -1109,7 +1118,40 @@ inline int launch_presums(const Plan* P, int dtype, const PackLayout& Lo, const 
     sb.cols = Lo.ns;
     sb.side = 1;
     sb.vec = ((uintptr_t)B & 15) == 0 && (ldb & 1) == 0 && (Lo.ks & 1) == 0 && ((uintptr_t)T & 15) == 0;
-    presum_kernel<<<gb, dim3(kEwThreads), 0, stream>>>(sb);
+    // one pass over each operand block for all products when its blocks fit
+    // the LDS of a workgroup; else one pass per product
+    for (int side = 0; side < 2; ++side) {
+      const SumJob& sj = side ? sb : sa;
+      const int nbr = side ? P->Kx : P->Mx, nbc = side ? P->Nx : P->Kx;
+      if (presum_all_enabled() && nbr * nbc <= kSaMaxBlocks) {
+        SumAllJob q{};
+        q.X = sj.X;
+        q.Y = sj.Y;
+        q.ldx = sj.ldx;
+        q.ldy = sj.ldy;
+        q.rows = sj.rows;
+        q.cols = sj.cols;
+        q.ny = sj.ny;
+        q.side = sj.side;
+        q.vec = sj.vec;
+        q.skip1 = sj.skip1;
+        q.nbr = nbr;
+        q.nbc = nbc;
+        q.nprod = P->R;
+        q.prods = sj.prods;
+        q.terms = sj.terms;
+        const dim3 g(tiles(q.rows, kEwRows) * q.cols);
+        const size_t lds = (size_t)nbr * nbc * 64 * sizeof(f64x2);
+        static const hipError_t attr = hipFuncSetAttribute(
+            reinterpret_cast<const void*>(presum_all_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+            kSaMaxBlocks * 64 * (int)sizeof(f64x2));
+        if (attr != hipSuccess) return kEHip;
+        hipLaunchKernelGGL(presum_all_kernel, g, dim3(64 * kSaWaves), lds, stream, q);
+      } else {
+        presum_kernel<<<side ? gb : ga, dim3(kEwThreads), 0, stream>>>(sj);
+      }
+      if (hipGetLastError() != hipSuccess) return kEHip;
+    }
   } else {
     SumJobL sa{};
     sa.X = A;
