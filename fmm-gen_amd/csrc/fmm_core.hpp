@@ -1475,6 +1475,98 @@ __global__ void __launch_bounds__(kEwThreads) madd_kernel(AddJob A) {
   }
 }
 
+// M_Add in one pass over M: a one-wave workgroup keeps one column run
+// (128 rows) of every C block in LDS, walks the products in order reading
+// each M_r run once (kMaPf products ahead), applies C_b = fma(w, M_r, C_b)
+// for each C term (b, w) of product r, and writes the C runs back once.
+// Per C block the updates come in product order -- the sequence of
+// madd_kernel -- so the results agree bit for bit, while each M_r element
+// is read once instead of once per W term (333: 23 M reads per element
+// instead of 45; <2,2,2> x <3,2,3>: 105 instead of 360).  Product r's C
+// terms are rblk / rw [rptr[r], rptr[r+1]); the k-fringe product R + b
+// (present when nlive > R) feeds block b alone, with weight 1.
+// Dynamic LDS: 1 KiB per C block.
+#ifndef FMM_MADD_PF
+#define FMM_MADD_PF 16
+#endif
+constexpr int kMaPf = FMM_MADD_PF;
+constexpr int kMaMaxBlocks = 64;
+struct AddAllJob {
+  const double* M;
+  double* C;
+  long long ldm, ldc;
+  int rows, cols;      // leaf block extent (ms x ns)
+  int mx, nx;          // M blocks per block column; C blocks per block row
+  int ncb;             // C blocks (Mx * Nx)
+  int nlive;           // products present in M
+  int vec;
+  const int* rptr;
+  const int* rblk;
+  const double* rw;
+};
+
+__global__ void __launch_bounds__(64) madd_all_kernel(AddAllJob A) {
+  extern __shared__ f64x2 ma_lds[];
+  const int tr = (A.rows + 127) / 128;
+  const int t = (int)blockIdx.x;
+  const int r0 = (t % tr) * 128;
+  const int c = t / tr;
+  const int lane = threadIdx.x;
+  const int i = r0 + 2 * lane;
+  const bool v2 = A.vec && i + 1 < A.rows;
+  auto ld2 = [&](gcdouble* p) {
+    f64x2 v = f64x2{0.0, 0.0};
+    if (v2) {
+      v = *(gcf64x2*)(p + i);
+    } else {
+      if (i < A.rows) v.x = p[i];
+      if (i + 1 < A.rows) v.y = p[i + 1];
+    }
+    return v;
+  };
+  auto cblk = [&](int b) {
+    return gptr(A.C) + (long long)(b / A.nx) * A.rows + (long long)(b % A.nx) * A.cols * A.ldc +
+           (long long)c * A.ldc;
+  };
+  auto mblk = [&](int r) {
+    return (gcdouble*)A.M + (long long)(r % A.mx) * A.rows + (long long)(r / A.mx) * A.cols * A.ldm +
+           (long long)c * A.ldm;
+  };
+  for (int b = 0; b < A.ncb; ++b) ma_lds[b * 64 + lane] = ld2(cblk(b));
+  f64x2 mr[kMaPf];
+#pragma unroll
+  for (int q = 0; q < kMaPf; ++q) mr[q] = q < A.nlive ? ld2(mblk(q)) : f64x2{0.0, 0.0};
+  for (int rb = 0; rb < A.nlive; rb += kMaPf) {
+#pragma unroll
+    for (int q = 0; q < kMaPf; ++q) {
+      const int r = rb + q;
+      if (r < A.nlive) {
+        const f64x2 m = mr[q];
+        if (r + kMaPf < A.nlive) mr[q] = ld2(mblk(r + kMaPf));
+        const int l1 = A.rptr[r + 1];
+        for (int l = A.rptr[r]; l < l1; ++l) {
+          const double w = A.rw[l];
+          f64x2* cv = ma_lds + A.rblk[l] * 64 + lane;
+          f64x2 x = *cv;
+          x.x = __builtin_fma(w, m.x, x.x);
+          x.y = __builtin_fma(w, m.y, x.y);
+          *cv = x;
+        }
+      }
+    }
+  }
+  for (int b = 0; b < A.ncb; ++b) {
+    const f64x2 v = ma_lds[b * 64 + lane];
+    gdouble* p = cblk(b);
+    if (v2) {
+      *(__attribute__((address_space(1))) f64x2*)(p + i) = v;
+    } else {
+      if (i < A.rows) p[i] = v.x;
+      if (i + 1 < A.rows) p[i + 1] = v.y;
+    }
+  }
+}
+
 // Records the XCC id of each workgroup (one launch per device at set-up):
 // the host enables XCD-local hand-over only if the ids look like an 8-XCD part
 // (workgroup i on XCC i % 8 for a fresh launch).

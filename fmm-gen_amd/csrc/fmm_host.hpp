@@ -125,6 +125,10 @@ struct Plan {
   // product order, with their W coefficients (the M_Add lists).
   DevVec<int> cptr, cprod;
   DevVec<double> cw;
+  // The same terms product by product: for product r (R + b: the k-fringe
+  // product of C block b) its C blocks and W coefficients (one-pass M_Add).
+  DevVec<int> rptr, rblk;
+  DevVec<double> rw;
   std::mutex mu;
 };
 
@@ -342,6 +346,23 @@ inline int build_plan(Plan* P, int nlev, const int* dims, int R, const double* U
     P->cw.h.push_back(1.0);
     P->cptr.h.push_back((int)P->cprod.h.size());
   }
+  P->rptr.h.assign(1, 0);
+  P->rblk.h.clear();
+  P->rw.h.clear();
+  for (int r = 0; r < R; ++r) {
+    const Prod& o = prods[r];
+    for (int t = 0; t < o.nc; ++t) {
+      const Term& c = terms[o.c0 + t];
+      P->rblk.h.push_back(c.bi * P->Nx + c.bj);
+      P->rw.h.push_back(c.coef);
+    }
+    P->rptr.h.push_back((int)P->rblk.h.size());
+  }
+  for (int cb = 0; cb < P->Mx * P->Nx; ++cb) {
+    P->rblk.h.push_back(cb);
+    P->rw.h.push_back(1.0);
+    P->rptr.h.push_back((int)P->rblk.h.size());
+  }
   return kOK;
 }
 
@@ -366,6 +387,24 @@ inline int device_clists(Plan* P, int** ptr, int** prod, double** w) {
   return rc;
 }
 
+inline int device_rlists(Plan* P, int** ptr, int** blk, double** w) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return kEHip;
+  std::lock_guard<std::mutex> g(P->mu);
+  int rc = P->rptr.get(dev, ptr);
+  if (rc == kOK) rc = P->rblk.get(dev, blk);
+  if (rc == kOK) rc = P->rw.get(dev, w);
+  return rc;
+}
+
+inline bool madd_all_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("FMM_MADD_ALL");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 inline void release_plan(Plan* P) {
   for (Table* T : {&P->main, &P->diag, &P->diag_by[0], &P->diag_by[1], &P->diag_by[2], &P->diag_by[3], &P->naive,
                    &P->ab, &P->naive_cs}) {
@@ -375,6 +414,9 @@ inline void release_plan(Plan* P) {
   P->cptr.release();
   P->cprod.release();
   P->cw.release();
+  P->rptr.release();
+  P->rblk.release();
+  P->rw.release();
 }
 
 // The plain-GEMM plan: one product, A(0,0) B(0,0) -> C(0,0).
@@ -1429,7 +1471,35 @@ inline int launch_pack(Plan* P, int pack, int dtype, const PackLayout& Lo, int m
   rc = launch_jobs_dtype(dtype, JL, naive ? 1 : P->max_nnz, nullptr, stream, naive);
   if (rc != kOK) return rc;
   const dim3 gm(ew_grid(Lo.ms, Lo.ns), P->Mx * P->Nx);
-  if (dtype == kF64) {
+  const int ncb = P->Mx * P->Nx;
+  if (dtype == kF64 && madd_all_enabled() && ncb <= kMaMaxBlocks) {
+    int *rptr, *rblk;
+    double* rw;
+    rc = device_rlists(P, &rptr, &rblk, &rw);
+    if (rc != kOK) return rc;
+    AddAllJob ad{};
+    ad.M = static_cast<const double*>(M);
+    ad.C = static_cast<double*>(C);
+    ad.ldm = Lo.ldm;
+    ad.ldc = ldc;
+    ad.rows = Lo.ms;
+    ad.cols = Lo.ns;
+    ad.mx = 1;
+    ad.nx = P->Nx;
+    ad.ncb = ncb;
+    ad.nlive = Lo.nlive;
+    ad.vec = ((uintptr_t)C & 15) == 0 && (ldc & 1) == 0 && ((uintptr_t)M & 15) == 0;
+    ad.rptr = rptr;
+    ad.rblk = rblk;
+    ad.rw = rw;
+    const size_t lds = (size_t)ncb * 1024;
+    static const hipError_t attr =
+        hipFuncSetAttribute(reinterpret_cast<const void*>(madd_all_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kMaMaxBlocks * 1024);
+    if (attr != hipSuccess) return kEHip;
+    const long long nwg = (long long)tiles(Lo.ms, 128) * Lo.ns;
+    hipLaunchKernelGGL(madd_all_kernel, dim3((unsigned)nwg), dim3(64), lds, stream, ad);
+  } else if (dtype == kF64) {
     AddJob ad{};
     ad.M = static_cast<const double*>(M);
     ad.C = static_cast<double*>(C);
