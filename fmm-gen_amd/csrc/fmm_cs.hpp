@@ -44,6 +44,8 @@ struct CsJob {
   int nprod;                // products, the k-fringe ones last
   int mx, nx;               // C blocks: mx x nx
   int vec;                  // bit0/1/2: A/B/C allow 16-byte accesses
+  int nmain;                // products of the leaf depth ks (the k-fringe ones follow)
+  int nterms;               // entries of the term table
   int pad;
   const Prod* prods;
   const Term* terms;

@@ -20,6 +20,7 @@
 #include "fmm_lowp.hpp"
 #include "fmm_h16d.hpp"
 #include "fmm_f64d.hpp"
+#include "fmm_csd.hpp"
 
 namespace fmm {
 
@@ -1011,9 +1012,49 @@ inline int launch_cs_k(CsJob J, hipStream_t stream) {
 
 // Single-term products (Naive over S / T, or single-term A / B blocks) by
 // C-block count: <= 4 blocks 128 x 64 positions, <= 9 blocks 64 x 64.
+template <class Cfg>
+inline int launch_csd_k(CsJob J, hipStream_t stream) {
+  DevInfo di;
+  int rc = dev_info(reinterpret_cast<const void*>(&csd_f64_kernel<Cfg>), Cfg::NT, &di);
+  if (rc != kOK) return rc;
+  J.tiles_m = tiles(J.ms, Cfg::TM);
+  J.tiles_n = tiles(J.ns, Cfg::TN);
+  const int units = J.tiles_m * J.tiles_n;
+  if (units <= 0 || J.nprod <= 0) return kOK;
+  int grid = std::max(1, std::min(units, di.cus * di.blocks_per_cu));
+  int nq = 1;
+  if (grid >= 8 * kMaxQueues) {
+    grid -= grid % kMaxQueues;
+    nq = kMaxQueues;
+  }
+  LaunchTiming& tm = launch_timing();
+  if (tm.on && tm.ev[0]) (void)hipEventRecord(tm.ev[0], stream);
+  hipLaunchKernelGGL(csd_f64_kernel<Cfg>, dim3(grid), dim3(Cfg::NT), 0, stream, J, nq);
+  if (hipGetLastError() != hipSuccess) return kEHip;
+  if (tm.on && tm.ev[1]) (void)hipEventRecord(tm.ev[1], stream);
+  return kOK;
+}
+
+// FMM_CSD=0 selects the register-staged C-stationary kernel (read per call).
+inline bool csd_enabled() {
+  const char* e = getenv("FMM_CSD");
+  return !(e && e[0] == '0');
+}
+
+#ifndef FMM_CSD4_CFG
+#define FMM_CSD4_CFG 4, 32, 3
+#endif
+#ifndef FMM_CSD9_CFG
+#define FMM_CSD9_CFG 9, 32, 4
+#endif
+
 inline int launch_cs(const CsJob& J, hipStream_t stream) {
   const int ncb = J.mx * J.nx;
   if (ncb > kCsMaxBlocks) return kEInval;
+  if (csd_enabled() && J.nprod <= kCsdMaxProd && J.nterms <= kCsdMaxTerms) {
+    if (ncb <= 4) return launch_csd_k<CsdCfg<FMM_CSD4_CFG>>(J, stream);
+    return launch_csd_k<CsdCfg<FMM_CSD9_CFG>>(J, stream);
+  }
   if (ncb <= 4) return launch_cs_k<CsCfg<4, 32, 1>>(J, stream);
   return launch_cs_k<CsCfg<9, 32, 1>>(J, stream);
 }
@@ -1034,6 +1075,7 @@ inline CsJob cs_core_job(const Plan& P, const Prod* dp, const Term* dt, int m, i
   j.ks = kd / P.Kx;
   j.kr = k - kd;
   j.nprod = P.R + (j.kr > 0 ? P.Mx * P.Nx : 0);
+  j.nmain = P.R;
   j.mx = P.Mx;
   j.nx = P.Nx;
   j.vec = vec_flags(kGeoF64, A, lda, B, ldb, j.ks, C, ldc);
@@ -1371,6 +1413,7 @@ inline int launch_naive_cs(Plan* P, const PackLayout& Lo, int m, int n, int k, c
   J.T = static_cast<const double*>(T);
   J.lds = Lo.lds;
   J.ldt = Lo.ldt;
+  J.nterms = (int)P->naive_cs.terms.h.size();
   return launch_cs(J, stream);
 }
 

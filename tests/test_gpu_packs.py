@@ -160,6 +160,68 @@ def test_abc_round_kernel_bitwise(torch, fmm, rt, oracle, spec):
         assert np.array_equal(goti, ref)
 
 
+CS_SPECS = [(["222"], [1]), (["232"], [1]), (["323"], [1]), (["333"], [1])]
+
+
+def sign_algorithm(fmm):
+    """<1,2,1> with two one-term products whose operands carry -1:
+    M1 = (-A0) B0, M2 = A1 (-B1), C = -M1 - M2 = A0 B0 + A1 B1."""
+    from fractions import Fraction as Fr
+    return fmm.Algorithm(name="sign121", levels=[(1, 2, 1)], U=[[Fr(-1), Fr(0)], [Fr(0), Fr(1)]],
+                         V=[[Fr(1), Fr(0)], [Fr(0), Fr(-1)]], W=[[Fr(-1), Fr(-1)]])
+
+
+@pytest.mark.parametrize("spec", CS_SPECS, ids=["222L1", "232L1", "323L1", "333L1"])
+def test_cs_direct_kernel_bitwise(torch, fmm, rt, oracle, spec):
+    """The C-stationary kernel with direct-to-LDS loads against the register-
+    staged one (FMM_CSD=0), bitwise on random inputs: leaf depths that end in
+    a partial stage of a multiple of 4 (direct, fewer k-steps) or not
+    (register-staged), k-fringe products, leaf rows / columns that leave edge
+    positions, odd leading dimensions (A / B blocks register-staged while the
+    pre-sums stay direct); exact on integer inputs against numpy and the
+    oracle."""
+    alg = fmm.build_algorithm(*spec)
+    plan = rt.Plan(alg)
+    Mx, Kx, Nx = alg.dims
+    rng = np.random.default_rng(22)
+    for (m, n, k, pad) in [(Mx * 8 * 20, Nx * 4 * 40, Kx * 48, 0),
+                           (Mx * 8 * 17 + 6, Nx * 4 * 29 + 1, Kx * 96 + 2, 0),
+                           (Mx * 136, Nx * 132, Kx * 240, 0),
+                           (Mx * 8 * 9 + 2, Nx * 4 * 11, Kx * 146, 1),
+                           (Mx * 8 * 5, Nx * 4 * 13 + 3, Kx * 50 + 1, 3)]:
+        A = rng.uniform(-1, 1, (m, k))
+        B = rng.uniform(-1, 1, (k, n))
+        C = rng.uniform(-1, 1, (m, n))
+        got = run(torch, rt, plan, "naive-cs", A, B, C, ldpad=pad)
+        with env("FMM_CSD", "0"):
+            want = run(torch, rt, plan, "naive-cs", A, B, C, ldpad=pad)
+        assert np.array_equal(got, want), (spec, m, n, k, pad, float(np.abs(got - want).max()))
+        Ai, Bi, Ci = int_mats(rng, m, n, k)
+        goti = run(torch, rt, plan, "naive-cs", Ai, Bi, Ci, ldpad=pad)
+        assert np.array_equal(goti, Ci + Ai @ Bi), (spec, m, n, k, pad)
+        ref = Ci.copy(order="F")
+        oracle.fmm(alg, Ai, Bi, ref, "naive")
+        assert np.array_equal(goti, ref)
+
+
+def test_cs_direct_kernel_signs(torch, fmm, rt):
+    """Single-term products whose operands carry -1 take the direct path with
+    the sign folded into their W coefficients: bitwise equal to the
+    register-staged kernel, which scales the operands."""
+    plan = rt.Plan(sign_algorithm(fmm))
+    rng = np.random.default_rng(23)
+    for (m, n, k, pad) in [(512, 512, 256, 0), (264, 300, 130, 0), (200, 136, 96, 3)]:
+        A = rng.uniform(-1, 1, (m, k))
+        B = rng.uniform(-1, 1, (k, n))
+        C = rng.uniform(-1, 1, (m, n))
+        got = run(torch, rt, plan, "naive-cs", A, B, C, ldpad=pad)
+        with env("FMM_CSD", "0"):
+            want = run(torch, rt, plan, "naive-cs", A, B, C, ldpad=pad)
+        assert np.array_equal(got, want), (m, n, k, pad)
+        Ai, Bi, Ci = int_mats(rng, m, n, k)
+        assert np.array_equal(run(torch, rt, plan, "naive-cs", Ai, Bi, Ci, ldpad=pad), Ci + Ai @ Bi)
+
+
 @pytest.mark.parametrize("pack", PACKS)
 def test_pack_strided_leading_dims(torch, fmm, rt, pack):
     """Odd leading dimensions and odd leaf depth take the element-wise paths
