@@ -1340,6 +1340,7 @@ __global__ void __launch_bounds__(kEwThreads) presum_kernel(SumJob S) {
 // as presum_kernel, so the sums are equal bit for bit.  Dynamic LDS: 1 KiB
 // per source block.
 constexpr int kSaWaves = 4;
+constexpr int kSaBatch = 8;
 constexpr int kSaMaxBlocks = 128;
 struct SumAllJob {
   const double* X;
@@ -1363,31 +1364,48 @@ __global__ void __launch_bounds__(64 * kSaWaves) presum_all_kernel(SumAllJob S) 
   const int t = (int)blockIdx.x;
   const int r0 = (t % tr) * kEwRows;
   const int c = t / tr;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = r0 + 2 * lane;
   const bool v2 = S.vec && i + 1 < S.rows;
   const int nb = S.nbr * S.nbc;
-  for (int b = wave; b < nb; b += kSaWaves) {
-    const int bi = b / S.nbc, bj = b - bi * S.nbc;
-    gcdouble* x = (gcdouble*)S.X + (long long)bi * S.rows + (long long)bj * S.cols * S.ldx + (long long)c * S.ldx;
-    f64x2 v = f64x2{0.0, 0.0};
-    if (v2) {
-      v = *(gcf64x2*)(x + i);
-    } else {
-      if (i < S.rows) v.x = x[i];
-      if (i + 1 < S.rows) v.y = x[i + 1];
+  // the wave's source blocks, kSaBatch loads in flight at a time (clamped
+  // indices past the last block load a valid block, not stored)
+  for (int b0 = wave; b0 < nb; b0 += kSaWaves * kSaBatch) {
+    f64x2 v[kSaBatch];
+#pragma unroll
+    for (int q = 0; q < kSaBatch; ++q) {
+      const int b = min(b0 + q * kSaWaves, nb - 1);
+      const int bi = b / S.nbc, bj = b - bi * S.nbc;
+      gcdouble* x = (gcdouble*)S.X + (long long)bi * S.rows + (long long)bj * S.cols * S.ldx + (long long)c * S.ldx;
+      v[q] = f64x2{0.0, 0.0};
+      if (v2) {
+        v[q] = *(gcf64x2*)(x + i);
+      } else {
+        if (i < S.rows) v[q].x = x[i];
+        if (i + 1 < S.rows) v[q].y = x[i + 1];
+      }
     }
-    sa_lds[b * 64 + lane] = v;
+#pragma unroll
+    for (int q = 0; q < kSaBatch; ++q) {
+      const int b = b0 + q * kSaWaves;
+      if (b < nb) sa_lds[b * 64 + lane] = v[q];
+    }
   }
   __syncthreads();
+  // product and term records as scalar loads (the stores below would
+  // otherwise make the compiler read them as vector loads, each waited for)
+  const auto* prods = (const __attribute__((address_space(4))) Prod*)S.prods;
+  const auto* terms = (const __attribute__((address_space(4))) Term*)S.terms;
   for (int r = wave; r < S.nprod; r += kSaWaves) {
-    const Prod* pp = S.prods + r;
-    const int t0 = S.side ? pp->b0 : pp->a0;
-    const int nt = S.side ? pp->nb : pp->na;
+    const int t0 = S.side ? prods[r].b0 : prods[r].a0;
+    const int nt = S.side ? prods[r].nb : prods[r].na;
     if (S.skip1 && nt == 1) continue;
     double s0 = 0.0, s1 = 0.0;
     for (int u = 0; u < nt; ++u) {
-      const Term tm = S.terms[t0 + u];
+      Term tm;
+      tm.coef = terms[t0 + u].coef;
+      tm.bi = terms[t0 + u].bi;
+      tm.bj = terms[t0 + u].bj;
       const f64x2 x = sa_lds[(tm.bi * S.nbc + tm.bj) * 64 + lane];
       s0 = u == 0 ? tm.coef * x.x : __builtin_fma(tm.coef, x.x, s0);
       s1 = u == 0 ? tm.coef * x.y : __builtin_fma(tm.coef, x.y, s1);
@@ -1486,10 +1504,6 @@ __global__ void __launch_bounds__(kEwThreads) madd_kernel(AddJob A) {
 // terms are rblk / rw [rptr[r], rptr[r+1]); the k-fringe product R + b
 // (present when nlive > R) feeds block b alone, with weight 1.
 // Dynamic LDS: 1 KiB per C block.
-#ifndef FMM_MADD_PF
-#define FMM_MADD_PF 16
-#endif
-constexpr int kMaPf = FMM_MADD_PF;
 constexpr int kMaMaxBlocks = 64;
 struct AddAllJob {
   const double* M;
@@ -1498,13 +1512,19 @@ struct AddAllJob {
   int rows, cols;      // leaf block extent (ms x ns)
   int mx, nx;          // M blocks per block column; C blocks per block row
   int ncb;             // C blocks (Mx * Nx)
-  int nlive;           // products present in M
+  int nlive;           // products present in M (>= 1)
   int vec;
   const int* rptr;
   const int* rblk;
   const double* rw;
 };
 
+// VEC: 16-byte accesses (C, M aligned, even leading dimensions; rows, a
+// multiple of 8, is even).  Every load is issued unconditionally -- lanes
+// past the block and prefetches past the last product read a clamped valid
+// address, never used -- so each wave has a fixed number of loads in flight
+// and the waits before each use count only the older ones.
+template <int kMaPf, bool VEC>
 __global__ void __launch_bounds__(64) madd_all_kernel(AddAllJob A) {
   extern __shared__ f64x2 ma_lds[];
   const int tr = (A.rows + 127) / 128;
@@ -1513,14 +1533,15 @@ __global__ void __launch_bounds__(64) madd_all_kernel(AddAllJob A) {
   const int c = t / tr;
   const int lane = threadIdx.x;
   const int i = r0 + 2 * lane;
-  const bool v2 = A.vec && i + 1 < A.rows;
+  const int ia = VEC ? min(i, A.rows - 2) : min(i, A.rows - 1);
+  const int ib = min(i + 1, A.rows - 1);
   auto ld2 = [&](gcdouble* p) {
-    f64x2 v = f64x2{0.0, 0.0};
-    if (v2) {
-      v = *(gcf64x2*)(p + i);
+    f64x2 v;
+    if (VEC) {
+      v = *(gcf64x2*)(p + ia);
     } else {
-      if (i < A.rows) v.x = p[i];
-      if (i + 1 < A.rows) v.y = p[i + 1];
+      v.x = p[ia];
+      v.y = p[ib];
     }
     return v;
   };
@@ -1535,14 +1556,14 @@ __global__ void __launch_bounds__(64) madd_all_kernel(AddAllJob A) {
   for (int b = 0; b < A.ncb; ++b) ma_lds[b * 64 + lane] = ld2(cblk(b));
   f64x2 mr[kMaPf];
 #pragma unroll
-  for (int q = 0; q < kMaPf; ++q) mr[q] = q < A.nlive ? ld2(mblk(q)) : f64x2{0.0, 0.0};
+  for (int q = 0; q < kMaPf; ++q) mr[q] = ld2(mblk(min(q, A.nlive - 1)));
   for (int rb = 0; rb < A.nlive; rb += kMaPf) {
 #pragma unroll
     for (int q = 0; q < kMaPf; ++q) {
       const int r = rb + q;
+      const f64x2 m = mr[q];
+      mr[q] = ld2(mblk(min(r + kMaPf, A.nlive - 1)));
       if (r < A.nlive) {
-        const f64x2 m = mr[q];
-        if (r + kMaPf < A.nlive) mr[q] = ld2(mblk(r + kMaPf));
         const int l1 = A.rptr[r + 1];
         for (int l = A.rptr[r]; l < l1; ++l) {
           const double w = A.rw[l];
@@ -1558,7 +1579,7 @@ __global__ void __launch_bounds__(64) madd_all_kernel(AddAllJob A) {
   for (int b = 0; b < A.ncb; ++b) {
     const f64x2 v = ma_lds[b * 64 + lane];
     gdouble* p = cblk(b);
-    if (v2) {
+    if (VEC && i + 1 < A.rows) {
       *(__attribute__((address_space(1))) f64x2*)(p + i) = v;
     } else {
       if (i < A.rows) p[i] = v.x;

@@ -26,19 +26,32 @@
 //   * Geometry by C-block count as in cs_f64_kernel: <= 4 blocks 128 x 64
 //     positions (8 waves of 32 x 32), <= 9 blocks 64 x 64 (8 waves of
 //     32 x 16); the position tile of every C block lives in registers.
+//
+// Measured at 14400^2 x 480 (Naive, the whole call): <2,2,2> 4.94 -> 4.63 ms
+// against the register-staged kernel, <2,3,2> 4.82 -> 4.55, <3,3,3> 5.97 ->
+// 5.42, <3,2,3> 6.13 -> 5.49; the plain GEMM takes 3.56 ms.  The MFMAs run
+// about half the time: with the operand loads removed the kernel still
+// takes 90% as long, with the MFMAs removed 45-60%, so the stage's fixed
+// costs (barrier, next-stage issue, the position's C tiles) add to the
+// MFMA time rather than hide under it.  Variants that did not pay: 16-deep
+// stages (+25%), rings of 2 (-2%) or 4 stages, the last k-step's MFMAs
+// issued before the refill, and 4 waves with one per SIMD (64 x 32 / 32 x 32
+// wave tiles, the C tiles in AGPRs: +6-11%).
 #pragma once
 #include "fmm_cs.hpp"
 #include "fmm_f64d.hpp"
 
 namespace fmm {
 
-template <int NCB_, int BK_, int NST_>
+template <int NCB_, int BK_, int NST_, int NW_ = 8>
 struct CsdCfg {
   static constexpr int NCB = NCB_, BK = BK_, NST = NST_;
-  static constexpr int NT = 512, NW = 8;
+  static constexpr int NW = NW_, NT = NW * 64;
   static constexpr int TM = NCB <= 4 ? 128 : 64, TN = 64;
-  static constexpr int WAVES_M = TM / 32, WAVES_N = NW / WAVES_M;
-  static constexpr int WTM = 32, WTN = TN / WAVES_N;
+  // 8 waves (two per SIMD): wave tiles 32 x 32 / 32 x 16; 4 waves (one per
+  // SIMD, the register file of the SIMD to each): 64 x 32 / 32 x 32
+  static constexpr int WAVES_M = NW == 8 ? TM / 32 : 2, WAVES_N = NW / WAVES_M;
+  static constexpr int WTM = TM / WAVES_M, WTN = TN / WAVES_N;
   static constexpr int FM = WTM / 16, FN = WTN / 16;
   static constexpr int KS = BK / 4;             // MFMA k-steps per stage
   static constexpr int A_ROW = TM * 8;          // bytes per k row of A
@@ -54,7 +67,7 @@ struct CsdCfg {
   static constexpr int A_LD = A_BYTES / 1024 / NW;
   static constexpr int B_LD = B_BYTES / 1024 / NW;
   static constexpr int LPT = A_LD + B_LD;
-  static_assert(NCB <= 9 && FM >= 1 && FN >= 1 && WAVES_M * WAVES_N == NW, "wave layout");
+  static_assert(NCB <= 9 && FM >= 1 && FN >= 1 && WAVES_M * WAVES_N == NW && (NW == 8 || NW == 4), "wave layout");
   static_assert(BK == 16 || BK == 32, "B swizzle written for 8 or 16 chunks per column");
   static_assert(A_LD * NW * 1024 == A_BYTES && B_LD * NW * 1024 == B_BYTES, "whole wave loads");
   static_assert(NST >= 2 && (NST - 1) * LPT <= 63, "vmcnt range");
@@ -256,7 +269,8 @@ __device__ __forceinline__ void csd_stage_head(const char* st, int kv, const int
 }
 
 template <class C>
-__global__ void __launch_bounds__(C::NT) csd_f64_kernel(CsJob J, int nq) {
+__global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::NW == 4 ? 1 : 2, 2)))
+csd_f64_kernel(CsJob J, int nq) {
   constexpr int BK = C::BK, FM = C::FM, FN = C::FN, NCB = C::NCB, NST = C::NST;
   __shared__ __align__(16) char lds[C::LDS_BYTES];
   __shared__ CsdSrc s_src[kCsdMaxProd];

@@ -398,6 +398,16 @@ inline int device_rlists(Plan* P, int** ptr, int** blk, double** w) {
   return rc;
 }
 
+#ifndef FMM_MADD_PF_LO
+#define FMM_MADD_PF_LO 16
+#endif
+#ifndef FMM_MADD_PF_HI
+#define FMM_MADD_PF_HI 32
+#endif
+#ifndef FMM_MADD_PF_SPLIT
+#define FMM_MADD_PF_SPLIT 16
+#endif
+
 inline bool madd_all_enabled() {
   static const bool on = [] {
     const char* e = getenv("FMM_MADD_ALL");
@@ -1515,7 +1525,7 @@ inline int launch_pack(Plan* P, int pack, int dtype, const PackLayout& Lo, int m
   if (rc != kOK) return rc;
   const dim3 gm(ew_grid(Lo.ms, Lo.ns), P->Mx * P->Nx);
   const int ncb = P->Mx * P->Nx;
-  if (dtype == kF64 && madd_all_enabled() && ncb <= kMaMaxBlocks) {
+  if (dtype == kF64 && madd_all_enabled() && ncb <= kMaMaxBlocks && Lo.nlive > 0 && Lo.ms >= 2) {
     int *rptr, *rblk;
     double* rw;
     rc = device_rlists(P, &rptr, &rblk, &rw);
@@ -1536,12 +1546,21 @@ inline int launch_pack(Plan* P, int pack, int dtype, const PackLayout& Lo, int m
     ad.rblk = rblk;
     ad.rw = rw;
     const size_t lds = (size_t)ncb * 1024;
-    static const hipError_t attr =
-        hipFuncSetAttribute(reinterpret_cast<const void*>(madd_all_kernel),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, kMaMaxBlocks * 1024);
-    if (attr != hipSuccess) return kEHip;
-    const long long nwg = (long long)tiles(Lo.ms, 128) * Lo.ns;
-    hipLaunchKernelGGL(madd_all_kernel, dim3((unsigned)nwg), dim3(64), lds, stream, ad);
+    // Few C blocks: many workgroups per CU, 16 M runs in flight each; many
+    // C blocks (LDS-bound occupancy): deeper prefetch, registers to spare.
+    auto go = [&](auto kern) {
+      static_assert(sizeof(kern) > 0, "");
+      if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              kMaMaxBlocks * 1024) != hipSuccess)
+        return kEHip;
+      const long long nwg = (long long)tiles(Lo.ms, 128) * Lo.ns;
+      hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(64), lds, stream, ad);
+      return kOK;
+    };
+    const bool hi = ncb > FMM_MADD_PF_SPLIT;
+    if (ad.vec) rc = hi ? go(madd_all_kernel<FMM_MADD_PF_HI, true>) : go(madd_all_kernel<FMM_MADD_PF_LO, true>);
+    else rc = hi ? go(madd_all_kernel<FMM_MADD_PF_HI, false>) : go(madd_all_kernel<FMM_MADD_PF_LO, false>);
+    if (rc != kOK) return rc;
   } else if (dtype == kF64) {
     AddJob ad{};
     ad.M = static_cast<const double*>(M);
