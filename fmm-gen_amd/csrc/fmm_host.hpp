@@ -1266,21 +1266,31 @@ inline int launch_presums(const Plan* P, int dtype, const PackLayout& Lo, const 
     sb.rows = Lo.ks;
     sb.cols = Lo.ns;
     sb.side = 1;
+    // paired loads: two consecutive elements of a column in one access
+    const int es = (dtype == kBF16 || dtype == kF16) ? 2 : 4;
+    auto pairs = [&](const void* X, long long ldx, int rows) {
+      return ((uintptr_t)X % (2 * es)) == 0 && (ldx & 1) == 0 && (rows & 1) == 0 ? 1 : 0;
+    };
+    sa.vec = pairs(A, lda, Lo.ms);
+    sb.vec = pairs(B, ldb, Lo.ks);
+    auto two = [&](auto k0, auto k1) {
+      // k0: element-wise loads, k1: paired loads
+      if (sa.vec) hipLaunchKernelGGL(k1, ga, dim3(kEwThreads), 0, stream, sa);
+      else hipLaunchKernelGGL(k0, ga, dim3(kEwThreads), 0, stream, sa);
+      if (sb.vec) hipLaunchKernelGGL(k1, gb, dim3(kEwThreads), 0, stream, sb);
+      else hipLaunchKernelGGL(k0, gb, dim3(kEwThreads), 0, stream, sb);
+    };
     if (dtype == kBF16) {
-      presum_lowp_kernel<kInBF16><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
-      presum_lowp_kernel<kInBF16><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
+      two(presum_lowp_kernel<kInBF16, false, false>, presum_lowp_kernel<kInBF16, false, true>);
     } else if (dtype == kF16) {
-      presum_lowp_kernel<kInF16><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
-      presum_lowp_kernel<kInF16><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
+      two(presum_lowp_kernel<kInF16, false, false>, presum_lowp_kernel<kInF16, false, true>);
     } else if (dtype == kBF16x3 && h16_direct()) {
       // split planes for the direct kernel: hi at S (T), lo one plane later
       sa.plane = (long long)Lo.lds * Lo.ks * P->R;
       sb.plane = (long long)Lo.ldt * Lo.ns * P->R;
-      presum_lowp_kernel<kInF32, true><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
-      presum_lowp_kernel<kInF32, true><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
+      two(presum_lowp_kernel<kInF32, true, false>, presum_lowp_kernel<kInF32, true, true>);
     } else {
-      presum_lowp_kernel<kInF32><<<ga, dim3(kEwThreads), 0, stream>>>(sa);
-      presum_lowp_kernel<kInF32><<<gb, dim3(kEwThreads), 0, stream>>>(sb);
+      two(presum_lowp_kernel<kInF32, false, false>, presum_lowp_kernel<kInF32, false, true>);
     }
   }
   if (hipGetLastError() != hipSuccess) return kEHip;

@@ -817,7 +817,8 @@ struct SumJobL {
   int rows, cols;
   int ny;
   int side;
-  int pad0, pad1;
+  int vec;                  // 1: X aligned to two elements, even ldx and rows (VEC instance)
+  int pad1;
   const Prod* prods;
   const Term* terms;
   long long plane;          // split output: element offset of the lo plane
@@ -839,7 +840,7 @@ __device__ __forceinline__ void store_in(typename InT<IN>::T* p, float v) {
 // SPLIT (FP32 inputs): the sum is stored as two BF16 planes, hi = bf16(s)
 // at Y and lo = bf16(s - hi) at Y + plane -- the split the split-BF16
 // kernels apply to an FP32 operand in registers.
-template <int IN, bool SPLIT = false>
+template <int IN, bool SPLIT = false, bool VEC = false>
 __global__ void __launch_bounds__(kEwThreads) presum_lowp_kernel(SumJobL S) {
   typedef typename InT<IN>::T T;
   static_assert(!SPLIT || IN == kInF32, "split planes from FP32 sums");
@@ -847,23 +848,54 @@ __global__ void __launch_bounds__(kEwThreads) presum_lowp_kernel(SumJobL S) {
   int r0, c0;
   ew_tile(S.rows, S.cols, &r0, &c0);
   if (c0 >= S.cols) return;
-  const Prod* pp = S.prods + r;
-  const int t0 = S.side ? pp->b0 : pp->a0;
-  const int nt = S.side ? pp->nb : pp->na;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // product and term records as scalar loads; the wave index made uniform
+  // so that the column loop, and the record reads in it, are too
+  const auto* prods = (const __attribute__((address_space(4))) Prod*)S.prods;
+  const auto* terms = (const __attribute__((address_space(4))) Term*)S.terms;
+  const int t0 = S.side ? prods[r].b0 : prods[r].a0;
+  const int nt = S.side ? prods[r].nb : prods[r].na;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = r0 + 2 * lane;
   T* y = static_cast<T*>(S.Y) + (long long)(r % S.ny) * S.rows + (long long)(r / S.ny) * S.cols * S.ldy;
   for (int c = c0 + wave; c < min(c0 + kEwCols, S.cols); c += kEwThreads / 64) {
     float s0 = 0.f, s1 = 0.f;
-    for (int t = 0; t < nt; ++t) {
-      const Term tm = S.terms[t0 + t];
-      const T* x = static_cast<const T*>(S.X) + (long long)tm.bi * S.rows + (long long)tm.bj * S.cols * S.ldx +
-                   (long long)c * S.ldx;
-      const float u = (float)tm.coef;
-      const float x0 = i < S.rows ? in_to_f<IN>(*(const __attribute__((address_space(1))) T*)(x + i)) : 0.f;
-      const float x1 = i + 1 < S.rows ? in_to_f<IN>(*(const __attribute__((address_space(1))) T*)(x + i + 1)) : 0.f;
-      s0 = t == 0 ? u * x0 : __builtin_fmaf(u, x0, s0);
-      s1 = t == 0 ? u * x1 : __builtin_fmaf(u, x1, s1);
+    // up to 4 terms' loads in flight, summed in term order
+    for (int tb = 0; tb < nt; tb += 4) {
+      float x0[4], x1[4], u[4];
+      // every load is issued (terms past nt and rows past the block read a
+      // clamped valid address, masked below), so the waits count loads
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int tq = min(tb + q, nt - 1);
+        const int bi = terms[t0 + tq].bi, bj = terms[t0 + tq].bj;
+        u[q] = (float)terms[t0 + tq].coef;
+        const T* x = static_cast<const T*>(S.X) + (long long)bi * S.rows + (long long)bj * S.cols * S.ldx +
+                     (long long)c * S.ldx;
+        if constexpr (VEC) {
+          const int ip = min(i, S.rows - 2);
+          if constexpr (IN == kInF32) {
+            const float2 v = *(const __attribute__((address_space(1))) float2*)(x + ip);
+            x0[q] = v.x;
+            x1[q] = v.y;
+          } else {
+            const unsigned w = *(const __attribute__((address_space(1))) unsigned*)(x + ip);
+            x0[q] = in_to_f<IN>((T)(w & 0xffffu));
+            x1[q] = in_to_f<IN>((T)(w >> 16));
+          }
+        } else {
+          x0[q] = in_to_f<IN>(*(const __attribute__((address_space(1))) T*)(x + min(i, S.rows - 1)));
+          x1[q] = in_to_f<IN>(*(const __attribute__((address_space(1))) T*)(x + min(i + 1, S.rows - 1)));
+        }
+        if (i >= S.rows) x0[q] = 0.f;
+        if (i + 1 >= S.rows) x1[q] = 0.f;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (tb + q < nt) {
+          s0 = tb + q == 0 ? u[q] * x0[q] : __builtin_fmaf(u[q], x0[q], s0);
+          s1 = tb + q == 0 ? u[q] * x1[q] : __builtin_fmaf(u[q], x1[q], s1);
+        }
+      }
     }
     if constexpr (SPLIT) {
       const long long off = (long long)(r % S.ny) * S.rows + (long long)(r / S.ny) * S.cols * S.ldy +
@@ -875,12 +907,38 @@ __global__ void __launch_bounds__(kEwThreads) presum_lowp_kernel(SumJobL S) {
         hc[e] = __builtin_bit_cast(u16, h);
         hc[e + S.plane] = __builtin_bit_cast(u16, l);
       };
-      if (i < S.rows) put(i, s0);
-      if (i + 1 < S.rows) put(i + 1, s1);
+      if (VEC) {
+        // rows even, i even: both rows or neither; Y columns stay aligned
+        if (i < S.rows) {
+          const __bf16 h0 = (__bf16)s0, h1 = (__bf16)s1;
+          const __bf16 l0 = (__bf16)(s0 - (float)h0), l1 = (__bf16)(s1 - (float)h1);
+          *(__attribute__((address_space(1))) unsigned*)(hc + i) =
+              __builtin_bit_cast(u16, h0) | ((unsigned)__builtin_bit_cast(u16, h1) << 16);
+          *(__attribute__((address_space(1))) unsigned*)(hc + i + S.plane) =
+              __builtin_bit_cast(u16, l0) | ((unsigned)__builtin_bit_cast(u16, l1) << 16);
+        }
+      } else {
+        if (i < S.rows) put(i, s0);
+        if (i + 1 < S.rows) put(i + 1, s1);
+      }
     } else {
       T* yc = y + (long long)c * S.ldy;
-      if (i < S.rows) store_in<IN>(yc + i, s0);
-      if (i + 1 < S.rows) store_in<IN>(yc + i + 1, s1);
+      if (VEC && i < S.rows) {
+        if constexpr (IN == kInF32) {
+          *(__attribute__((address_space(1))) float2*)(yc + i) = float2{s0, s1};
+        } else if constexpr (IN == kInBF16) {
+          const __bf16 h0 = (__bf16)s0, h1 = (__bf16)s1;
+          *(__attribute__((address_space(1))) unsigned*)(yc + i) =
+              __builtin_bit_cast(u16, h0) | ((unsigned)__builtin_bit_cast(u16, h1) << 16);
+        } else {
+          const _Float16 h0 = (_Float16)s0, h1 = (_Float16)s1;
+          *(__attribute__((address_space(1))) unsigned*)(yc + i) =
+              __builtin_bit_cast(u16, h0) | ((unsigned)__builtin_bit_cast(u16, h1) << 16);
+        }
+      } else if (!VEC) {
+        if (i < S.rows) store_in<IN>(yc + i, s0);
+        if (i + 1 < S.rows) store_in<IN>(yc + i + 1, s1);
+      }
     }
   }
 }
