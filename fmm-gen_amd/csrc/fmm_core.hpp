@@ -1564,28 +1564,14 @@ __global__ void __launch_bounds__(64) madd_all_kernel(AddAllJob A) {
       const f64x2 m = mr[q];
       mr[q] = ld2(mblk(min(r + kMaPf, A.nlive - 1)));
       if (r < A.nlive) {
-        // a product's terms name distinct C blocks: up to 4 runs are read
-        // before any is written back (one LDS round trip per 4 terms)
         const int l1 = A.rptr[r + 1];
-        for (int l = A.rptr[r]; l < l1; l += 4) {
-          f64x2 x[4];
-          f64x2* cv[4];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            if (l + q < l1) {
-              cv[q] = ma_lds + A.rblk[l + q] * 64 + lane;
-              x[q] = *cv[q];
-            }
-          }
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            if (l + q < l1) {
-              const double w = A.rw[l + q];
-              x[q].x = __builtin_fma(w, m.x, x[q].x);
-              x[q].y = __builtin_fma(w, m.y, x[q].y);
-              *cv[q] = x[q];
-            }
-          }
+        for (int l = A.rptr[r]; l < l1; ++l) {
+          const double w = A.rw[l];
+          f64x2* cv = ma_lds + A.rblk[l] * 64 + lane;
+          f64x2 x = *cv;
+          x.x = __builtin_fma(w, m.x, x.x);
+          x.y = __builtin_fma(w, m.y, x.y);
+          *cv = x;
         }
       }
     }

@@ -874,9 +874,9 @@ __global__ void __launch_bounds__(kEwThreads) presum_lowp_kernel(SumJobL S) {
         if constexpr (VEC) {
           const int ip = min(i, S.rows - 2);
           if constexpr (IN == kInF32) {
-            const float2 v = *(const __attribute__((address_space(1))) float2*)(x + ip);
-            x0[q] = v.x;
-            x1[q] = v.y;
+            const unsigned long long v = *(const __attribute__((address_space(1))) unsigned long long*)(x + ip);
+            x0[q] = __uint_as_float((unsigned)v);
+            x1[q] = __uint_as_float((unsigned)(v >> 32));
           } else {
             const unsigned w = *(const __attribute__((address_space(1))) unsigned*)(x + ip);
             x0[q] = in_to_f<IN>((T)(w & 0xffffu));
@@ -925,7 +925,8 @@ __global__ void __launch_bounds__(kEwThreads) presum_lowp_kernel(SumJobL S) {
       T* yc = y + (long long)c * S.ldy;
       if (VEC && i < S.rows) {
         if constexpr (IN == kInF32) {
-          *(__attribute__((address_space(1))) float2*)(yc + i) = float2{s0, s1};
+          *(__attribute__((address_space(1))) unsigned long long*)(yc + i) =
+              (unsigned long long)__float_as_uint(s0) | ((unsigned long long)__float_as_uint(s1) << 32);
         } else if constexpr (IN == kInBF16) {
           const __bf16 h0 = (__bf16)s0, h1 = (__bf16)s1;
           *(__attribute__((address_space(1))) unsigned*)(yc + i) =
