@@ -1588,6 +1588,104 @@ __global__ void __launch_bounds__(64) madd_all_kernel(AddAllJob A) {
   }
 }
 
+// M_Add in one pass over M for many C blocks: a 4-wave workgroup keeps one
+// column run of every C block in LDS (shared), and wave w owns the blocks b
+// with b % 4 == w: it walks, in product order, only the products with a
+// term on its blocks (its own list: wprod, wptr, wblk, ww from wbeg[w]),
+// reading those M_r runs (kMaPf ahead) and applying their terms, then
+// writes its blocks back.  Each block sees the updates of madd_kernel in
+// the same order (bit for bit); against the one-wave kernel the workgroup
+// holds four times the waves per CU for the same LDS and each wave a
+// quarter of the read-modify-write chain.
+constexpr int kMa4Waves = 4;
+struct AddAll4Job {
+  const double* M;
+  double* C;
+  long long ldm, ldc;
+  int rows, cols;
+  int mx, nx;
+  int ncb;
+  int nlive;
+  const int* wbeg;
+  const int* wprod;
+  const int* wptr;
+  const int* wblk;
+  const double* ww;
+};
+
+template <int kMaPf, bool VEC>
+__global__ void __launch_bounds__(64 * kMa4Waves) madd_all4_kernel(AddAll4Job A) {
+  extern __shared__ f64x2 ma_lds[];
+  const int tr = (A.rows + 127) / 128;
+  const int t = (int)blockIdx.x;
+  const int r0 = (t % tr) * 128;
+  const int c = t / tr;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = r0 + 2 * lane;
+  const int ia = VEC ? min(i, A.rows - 2) : min(i, A.rows - 1);
+  const int ib = min(i + 1, A.rows - 1);
+  auto ld2 = [&](gcdouble* p) {
+    f64x2 v;
+    if (VEC) {
+      v = *(gcf64x2*)(p + ia);
+    } else {
+      v.x = p[ia];
+      v.y = p[ib];
+    }
+    return v;
+  };
+  auto cblk = [&](int b) {
+    return gptr(A.C) + (long long)(b / A.nx) * A.rows + (long long)(b % A.nx) * A.cols * A.ldc +
+           (long long)c * A.ldc;
+  };
+  auto mblk = [&](int r) {
+    return (gcdouble*)A.M + (long long)(r % A.mx) * A.rows + (long long)(r / A.mx) * A.cols * A.ldm +
+           (long long)c * A.ldm;
+  };
+  const auto* wbeg = (const __attribute__((address_space(4))) int*)A.wbeg;
+  const auto* wprod = (const __attribute__((address_space(4))) int*)A.wprod;
+  const auto* wptr = (const __attribute__((address_space(4))) int*)A.wptr;
+  const auto* wblk = (const __attribute__((address_space(4))) int*)A.wblk;
+  const auto* ww = (const __attribute__((address_space(4))) double*)A.ww;
+  for (int b = wave; b < A.ncb; b += kMa4Waves) ma_lds[b * 64 + lane] = ld2(cblk(b));
+  const int e0 = wbeg[wave], n = wbeg[wave + 1] - e0;
+  if (n > 0) {
+    auto prod_m = [&](int e) { return min(wprod[e0 + min(e, n - 1)], A.nlive - 1); };
+    f64x2 mr[kMaPf];
+#pragma unroll
+    for (int q = 0; q < kMaPf; ++q) mr[q] = ld2(mblk(prod_m(q)));
+    for (int eb = 0; eb < n; eb += kMaPf) {
+#pragma unroll
+      for (int q = 0; q < kMaPf; ++q) {
+        const int e = eb + q;
+        const f64x2 m = mr[q];
+        mr[q] = ld2(mblk(prod_m(e + kMaPf)));
+        if (e < n && wprod[e0 + e] < A.nlive) {
+          const int l1 = wptr[e0 + e + 1];
+          for (int l = wptr[e0 + e]; l < l1; ++l) {
+            const double w = ww[l];
+            const int ix = wblk[l] * 64 + lane;
+            f64x2 x = ma_lds[ix];
+            x.x = __builtin_fma(w, m.x, x.x);
+            x.y = __builtin_fma(w, m.y, x.y);
+            ma_lds[ix] = x;
+          }
+        }
+      }
+    }
+  }
+  for (int b = wave; b < A.ncb; b += kMa4Waves) {
+    const f64x2 v = ma_lds[b * 64 + lane];
+    gdouble* p = cblk(b);
+    if (VEC && i + 1 < A.rows) {
+      *(__attribute__((address_space(1))) f64x2*)(p + i) = v;
+    } else {
+      if (i < A.rows) p[i] = v.x;
+      if (i + 1 < A.rows) p[i + 1] = v.y;
+    }
+  }
+}
+
 // Records the XCC id of each workgroup (one launch per device at set-up):
 // the host enables XCD-local hand-over only if the ids look like an 8-XCD part
 // (workgroup i on XCC i % 8 for a fresh launch).

@@ -130,6 +130,12 @@ struct Plan {
   // product of C block b) its C blocks and W coefficients (one-pass M_Add).
   DevVec<int> rptr, rblk;
   DevVec<double> rw;
+  // The same split over the 4 waves of a multi-wave M_Add workgroup: wave w
+  // owns the C blocks b with b % 4 == w; its entries wbeg[w] .. wbeg[w+1]
+  // list, in product order, the products with a term on its blocks (wprod)
+  // and those terms (wptr into wblk / ww).
+  DevVec<int> wbeg, wprod, wptr, wblk;
+  DevVec<double> ww;
   std::mutex mu;
 };
 
@@ -364,6 +370,28 @@ inline int build_plan(Plan* P, int nlev, const int* dims, int R, const double* U
     P->rw.h.push_back(1.0);
     P->rptr.h.push_back((int)P->rblk.h.size());
   }
+  P->wbeg.h.assign(1, 0);
+  P->wprod.h.clear();
+  P->wptr.h.assign(1, 0);
+  P->wblk.h.clear();
+  P->ww.h.clear();
+  const int nr = (int)P->rptr.h.size() - 1;   // R + Mx*Nx products
+  for (int w = 0; w < kMa4Waves; ++w) {
+    for (int r = 0; r < nr; ++r) {
+      bool any = false;
+      for (int l = P->rptr.h[r]; l < P->rptr.h[r + 1]; ++l) {
+        if (P->rblk.h[l] % kMa4Waves != w) continue;
+        P->wblk.h.push_back(P->rblk.h[l]);
+        P->ww.h.push_back(P->rw.h[l]);
+        any = true;
+      }
+      if (any) {
+        P->wprod.h.push_back(r);
+        P->wptr.h.push_back((int)P->wblk.h.size());
+      }
+    }
+    P->wbeg.h.push_back((int)P->wprod.h.size());
+  }
   return kOK;
 }
 
@@ -388,6 +416,18 @@ inline int device_clists(Plan* P, int** ptr, int** prod, double** w) {
   return rc;
 }
 
+inline int device_wlists(Plan* P, int** beg, int** prod, int** ptr, int** blk, double** w) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return kEHip;
+  std::lock_guard<std::mutex> g(P->mu);
+  int rc = P->wbeg.get(dev, beg);
+  if (rc == kOK) rc = P->wprod.get(dev, prod);
+  if (rc == kOK) rc = P->wptr.get(dev, ptr);
+  if (rc == kOK) rc = P->wblk.get(dev, blk);
+  if (rc == kOK) rc = P->ww.get(dev, w);
+  return rc;
+}
+
 inline int device_rlists(Plan* P, int** ptr, int** blk, double** w) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return kEHip;
@@ -407,6 +447,13 @@ inline int device_rlists(Plan* P, int** ptr, int** blk, double** w) {
 #ifndef FMM_MADD_PF_SPLIT
 #define FMM_MADD_PF_SPLIT 16
 #endif
+
+// FMM_MADD_ALL4=0 keeps algorithms with many C blocks on the one-wave
+// M_Add kernel (read per call).
+inline bool madd_all4_enabled() {
+  const char* e = getenv("FMM_MADD_ALL4");
+  return !(e && e[0] == '0');
+}
 
 inline bool madd_all_enabled() {
   static const bool on = [] {
@@ -428,6 +475,11 @@ inline void release_plan(Plan* P) {
   P->rptr.release();
   P->rblk.release();
   P->rw.release();
+  P->wbeg.release();
+  P->wprod.release();
+  P->wptr.release();
+  P->wblk.release();
+  P->ww.release();
 }
 
 // The plain-GEMM plan: one product, A(0,0) B(0,0) -> C(0,0).
@@ -1568,8 +1620,42 @@ inline int launch_pack(Plan* P, int pack, int dtype, const PackLayout& Lo, int m
       return kOK;
     };
     const bool hi = ncb > FMM_MADD_PF_SPLIT;
-    if (ad.vec) rc = hi ? go(madd_all_kernel<FMM_MADD_PF_HI, true>) : go(madd_all_kernel<FMM_MADD_PF_LO, true>);
-    else rc = hi ? go(madd_all_kernel<FMM_MADD_PF_HI, false>) : go(madd_all_kernel<FMM_MADD_PF_LO, false>);
+    if (hi && madd_all4_enabled()) {
+      // many C blocks: the 4-wave kernel, each wave a quarter of the blocks
+      int *wbeg, *wprod, *wptr, *wblk;
+      double* ww;
+      rc = device_wlists(P, &wbeg, &wprod, &wptr, &wblk, &ww);
+      if (rc != kOK) return rc;
+      AddAll4Job a4{};
+      a4.M = ad.M;
+      a4.C = ad.C;
+      a4.ldm = ad.ldm;
+      a4.ldc = ad.ldc;
+      a4.rows = ad.rows;
+      a4.cols = ad.cols;
+      a4.mx = ad.mx;
+      a4.nx = ad.nx;
+      a4.ncb = ad.ncb;
+      a4.nlive = ad.nlive;
+      a4.wbeg = wbeg;
+      a4.wprod = wprod;
+      a4.wptr = wptr;
+      a4.wblk = wblk;
+      a4.ww = ww;
+      auto go4 = [&](auto kern) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                kMaMaxBlocks * 1024) != hipSuccess)
+          return kEHip;
+        const long long nwg = (long long)tiles(Lo.ms, 128) * Lo.ns;
+        hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(64 * kMa4Waves), lds, stream, a4);
+        return kOK;
+      };
+      rc = ad.vec ? go4(madd_all4_kernel<FMM_MADD_PF_LO, true>) : go4(madd_all4_kernel<FMM_MADD_PF_LO, false>);
+    } else if (ad.vec) {
+      rc = hi ? go(madd_all_kernel<FMM_MADD_PF_HI, true>) : go(madd_all_kernel<FMM_MADD_PF_LO, true>);
+    } else {
+      rc = hi ? go(madd_all_kernel<FMM_MADD_PF_HI, false>) : go(madd_all_kernel<FMM_MADD_PF_LO, false>);
+    }
     if (rc != kOK) return rc;
   } else if (dtype == kF64) {
     AddJob ad{};
