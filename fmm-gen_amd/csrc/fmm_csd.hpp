@@ -29,7 +29,7 @@
 //
 // Measured at 14400^2 x 480 (Naive, the whole call): <2,2,2> 4.94 -> 4.63 ms
 // against the register-staged kernel, <2,3,2> 4.82 -> 4.55, <3,3,3> 5.97 ->
-// 5.42, <3,2,3> 6.13 -> 5.49; the plain GEMM takes 3.56 ms.  The MFMAs run
+// 4.86, <3,2,3> 6.13 -> 5.04; the plain GEMM takes 3.56 ms.  The MFMAs run
 // about half the time: with the operand loads removed the kernel still
 // takes 90% as long, with the MFMAs removed 45-60%, so the stage's fixed
 // costs (barrier, next-stage issue, the position's C tiles) add to the
@@ -230,14 +230,14 @@ __device__ __forceinline__ void csd_frags(const char* st, int kk, const int (&ao
 // The MFMAs of one k-step (operands as in the ordered kernel, X = B^T,
 // Y = A^T); `ok` false zeroes this lane's fragments first (k past the valid
 // depth of a product's last stage, whose rows were loaded clamped).
-template <class C>
+template <class C, bool MASK = true>
 __device__ __forceinline__ void csd_mfma(const double (&a)[C::FM], const double (&b)[C::FN], bool ok,
                                          f64x4 (&acc)[C::FM][C::FN]) {
   double x[C::FM], y[C::FN];
 #pragma unroll
-  for (int mi = 0; mi < C::FM; ++mi) x[mi] = ok ? a[mi] : 0.0;
+  for (int mi = 0; mi < C::FM; ++mi) x[mi] = !MASK || ok ? a[mi] : 0.0;
 #pragma unroll
-  for (int ni = 0; ni < C::FN; ++ni) y[ni] = ok ? b[ni] : 0.0;
+  for (int ni = 0; ni < C::FN; ++ni) y[ni] = !MASK || ok ? b[ni] : 0.0;
 #pragma unroll
   for (int mi = 0; mi < C::FM; ++mi)
 #pragma unroll
@@ -252,7 +252,7 @@ __device__ __forceinline__ void csd_mfma(const double (&a)[C::FM], const double 
 // been read, and then issue the last k-step's MFMAs.  kv: valid k of the
 // stage (BK but in a product's partial last stage, whose further k-steps
 // contract zeros, as the register-staged kernels' zero fill does).
-template <class C>
+template <class C, bool MASK>
 __device__ __forceinline__ void csd_stage_head(const char* st, int kv, const int (&aofs)[C::FM], int bcol, int lk,
                                                f64x4 (&acc)[C::FM][C::FN], double (&al)[C::FM],
                                                double (&bl)[C::FN]) {
@@ -264,7 +264,7 @@ __device__ __forceinline__ void csd_stage_head(const char* st, int kv, const int
     if (kk + 1 < C::KS - 1) csd_frags<C>(st, kk + 1, aofs, bcol, lk, a[c ^ 1], b[c ^ 1]);
     else csd_frags<C>(st, kk + 1, aofs, bcol, lk, al, bl);
     __builtin_amdgcn_sched_barrier(0);
-    csd_mfma<C>(a[c], b[c], 4 * kk + lk < kv, acc);
+    csd_mfma<C, MASK>(a[c], b[c], 4 * kk + lk < kv, acc);
   }
 }
 
@@ -392,14 +392,21 @@ csd_f64_kernel(CsJob J, int nq) {
       for (int s = 0; s < ns_tot; ++s) {
         const int rem = min(BK, depth - ckb);
         double al[FM], bl[FN];
-        csd_stage_head<C>(stage(s), rem, aofs, bcol, lk, acc, al, bl);
+        // a product's partial last stage masks its fragments; the <= 9-block
+        // layout (two MFMAs per k-step) runs whole stages unmasked, the
+        // <= 4-block one keeps a single (masked) stage path, whose second
+        // copy would spill
+        const bool part = C::NCB <= 4 || rem < BK;
+        if (part) csd_stage_head<C, true>(stage(s), rem, aofs, bcol, lk, acc, al, bl);
+        else csd_stage_head<C, false>(stage(s), rem, aofs, bcol, lk, acc, al, bl);
         // every fragment of stage s is read: publish stage s+1, refill s's buffer
         if (s + 1 < ns_tot) {
           wait(s + 1, min(s + NST - 1, ns_tot - 1));
           __syncthreads();
           if (s + NST < ns_tot) fill(s + NST);
         }
-        csd_mfma<C>(al, bl, 4 * (C::KS - 1) + lk < rem, acc);
+        if (part) csd_mfma<C, true>(al, bl, 4 * (C::KS - 1) + lk < rem, acc);
+        else csd_mfma<C, false>(al, bl, true, acc);
         ckb += BK;
         if (ckb >= depth) {
           // product complete: C_j = fma(w_j, M, C_j) for its W terms in table
