@@ -29,12 +29,13 @@
 //
 // Measured at 14400^2 x 480 (Naive, the whole call): <2,2,2> 4.94 -> 4.63 ms
 // against the register-staged kernel, <2,3,2> 4.82 -> 4.55, <3,3,3> 5.97 ->
-// 4.86, <3,2,3> 6.13 -> 5.04; the plain GEMM takes 3.56 ms.  The MFMAs run
+// 4.67, <3,2,3> 6.13 -> 4.85 (rings of 2 stages; 3 and 4 measured 2-3%
+// slower); the plain GEMM takes 3.56 ms.  The MFMAs run
 // about half the time: with the operand loads removed the kernel still
 // takes 90% as long, with the MFMAs removed 45-60%, so the stage's fixed
 // costs (barrier, next-stage issue, the position's C tiles) add to the
 // MFMA time rather than hide under it.  Variants that did not pay: 16-deep
-// stages (+25%), rings of 2 (-2%) or 4 stages, the last k-step's MFMAs
+// stages (+25%), the last k-step's MFMAs
 // issued before the refill, and 4 waves with one per SIMD (64 x 32 / 32 x 32
 // wave tiles, the C tiles in AGPRs: +6-11%).
 #pragma once

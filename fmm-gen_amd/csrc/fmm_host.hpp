@@ -1104,10 +1104,10 @@ inline bool csd_enabled() {
 }
 
 #ifndef FMM_CSD4_CFG
-#define FMM_CSD4_CFG 4, 32, 3
+#define FMM_CSD4_CFG 4, 32, 2
 #endif
 #ifndef FMM_CSD9_CFG
-#define FMM_CSD9_CFG 9, 32, 4
+#define FMM_CSD9_CFG 9, 32, 2
 #endif
 
 inline int launch_cs(const CsJob& J, hipStream_t stream) {
