@@ -27,17 +27,17 @@
 //     positions (8 waves of 32 x 32), <= 9 blocks 64 x 64 (8 waves of
 //     32 x 16); the position tile of every C block lives in registers.
 //
-// Measured at 14400^2 x 480 (Naive, the whole call): <2,2,2> 4.94 -> 4.63 ms
-// against the register-staged kernel, <2,3,2> 4.82 -> 4.55, <3,3,3> 5.97 ->
-// 4.67, <3,2,3> 6.13 -> 4.85 (rings of 2 stages; 3 and 4 measured 2-3%
-// slower); the plain GEMM takes 3.56 ms.  The MFMAs run
-// about half the time: with the operand loads removed the kernel still
-// takes 90% as long, with the MFMAs removed 45-60%, so the stage's fixed
-// costs (barrier, next-stage issue, the position's C tiles) add to the
-// MFMA time rather than hide under it.  Variants that did not pay: 16-deep
-// stages (+25%), the last k-step's MFMAs
-// issued before the refill, and 4 waves with one per SIMD (64 x 32 / 32 x 32
-// wave tiles, the C tiles in AGPRs: +6-11%).
+// Measured at 14400^2 x 480 (Naive, the whole call) against the register-
+// staged kernel: <2,2,2> 4.94 -> 4.51 ms, <2,3,2> 4.82 -> 4.33, <3,3,3>
+// 5.97 -> 4.67, <3,2,3> 6.13 -> 4.85 (rings of 2 stages, 3 and 4 measured
+// 2-3% slower; whole stages unmasked); the plain GEMM takes 3.56 ms.  The
+// MFMAs run about half the time: with the operand loads removed the kernel
+// still takes 90% as long, with the MFMAs removed 45-60%, so the stage's
+// fixed costs (barrier, next-stage issue, the position's C tiles) add to
+// the MFMA time rather than hide under it.  Variants that did not pay:
+// 16-deep stages (+25%), the last k-step's MFMAs issued before the refill,
+// and 4 waves with one per SIMD (64 x 32 / 32 x 32 wave tiles, the C tiles
+// in AGPRs: +6-11%).
 #pragma once
 #include "fmm_cs.hpp"
 #include "fmm_f64d.hpp"
@@ -393,11 +393,9 @@ csd_f64_kernel(CsJob J, int nq) {
       for (int s = 0; s < ns_tot; ++s) {
         const int rem = min(BK, depth - ckb);
         double al[FM], bl[FN];
-        // a product's partial last stage masks its fragments; the <= 9-block
-        // layout (two MFMAs per k-step) runs whole stages unmasked, the
-        // <= 4-block one keeps a single (masked) stage path, whose second
-        // copy would spill
-        const bool part = C::NCB <= 4 || rem < BK;
+        // a product's partial last stage masks its fragments; whole stages
+        // run unmasked (the masks cost the 32 x 16 wave tiles 10%)
+        const bool part = rem < BK;
         if (part) csd_stage_head<C, true>(stage(s), rem, aofs, bcol, lk, acc, al, bl);
         else csd_stage_head<C, false>(stage(s), rem, aofs, bcol, lk, acc, al, bl);
         // every fragment of stage s is read: publish stage s+1, refill s's buffer
