@@ -502,8 +502,8 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
-              "sharding of this path is described in DESIGN.md and not built.", flush=True)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); sharding a call over "
+              "several GPUs (by C tile position) is not built.", flush=True)
         sys.exit(2)
     import torch
     from __graft_entry__ import load_package
