@@ -222,6 +222,34 @@ def test_cs_direct_kernel_signs(torch, fmm, rt):
         assert np.array_equal(run(torch, rt, plan, "naive-cs", Ai, Bi, Ci, ldpad=pad), Ci + Ai @ Bi)
 
 
+@pytest.mark.parametrize("pack", ["ab", "naive"])
+def test_madd_many_blocks_bitwise(torch, fmm, rt, oracle, pack):
+    """M_Add of <2,2,2> x <3,2,3> (36 C blocks): the 4-wave kernel (each wave a
+    quarter of the C blocks, its own product list) against the one-wave
+    kernel (FMM_MADD_ALL4=0), bitwise on random inputs with edge rows and
+    columns, odd leading dimensions and a k-fringe; exact on integer inputs
+    against numpy and the oracle."""
+    alg = fmm.build_algorithm(["222", "323"], [1, 1])
+    plan = rt.Plan(alg)
+    Mx, Kx, Nx = alg.dims
+    rng = np.random.default_rng(24)
+    for (m, n, k, pad) in [(Mx * 8 * 17, Nx * 4 * 35, Kx * 71, 0),
+                           (Mx * 8 * 12 + 5, Nx * 4 * 24 + 2, Kx * 40 + 3, 1)]:
+        A = rng.uniform(-1, 1, (m, k))
+        B = rng.uniform(-1, 1, (k, n))
+        C = rng.uniform(-1, 1, (m, n))
+        got = run(torch, rt, plan, pack, A, B, C, ldpad=pad)
+        with env("FMM_MADD_ALL4", "0"):
+            want = run(torch, rt, plan, pack, A, B, C, ldpad=pad)
+        assert np.array_equal(got, want), (pack, m, n, k, pad, float(np.abs(got - want).max()))
+        Ai, Bi, Ci = int_mats(rng, m, n, k)
+        goti = run(torch, rt, plan, pack, Ai, Bi, Ci, ldpad=pad)
+        assert np.array_equal(goti, Ci + Ai @ Bi), (pack, m, n, k, pad)
+        ref = Ci.copy(order="F")
+        oracle.fmm(alg, Ai, Bi, ref, pack)
+        assert np.array_equal(goti, ref)
+
+
 @pytest.mark.parametrize("pack", PACKS)
 def test_pack_strided_leading_dims(torch, fmm, rt, pack):
     """Odd leading dimensions and odd leaf depth take the element-wise paths
