@@ -26,7 +26,11 @@
 // 66.3 -> 67.6-67.9 TFLOP/s, Naive 333 L1 67.1 -> 68.5-69.1, 16384^3 GEMM
 // 66.1 -> 67.8, rank-k 14400^2 x 480 GEMM 53.5 -> 55.6; MFMA busy 0.85 ->
 // 0.87.  16-deep stages in rings of 3 or 4 ran 3-4% slower than 32-deep
-// stages in a ring of 2.
+// stages in a ring of 2.  Letting the waves of an edge tile whose sub-tile
+// lies outside the leaf block skip their MFMAs (a branch around the stage's
+// fragment reads and MFMAs) saved 0.5% on <2,2,2> x <3,2,3> (leaf 1600, half
+// tiles at every edge) and cost 3-4% on whole tiles, with or without
+// pairing opposite row halves on one SIMD.
 #pragma once
 #include "fmm_h16d.hpp"
 
